@@ -1,0 +1,147 @@
+/*
+ * C-ABI of the MI355X Hartree-Fock engine (libpyfacc.so).
+ *
+ * Drop-in boundary: the first two entry points have exactly the signatures
+ * of the reference's shared library, so the reference's Python driver
+ * (src/python/helper_functions.py:90-149 via ctypes) and its example script
+ * (single_file.py:21-39) can load this library in place of libscf.so.
+ * The pf_* entry points are additions for tests, benchmarks and callers that
+ * want status codes, options and intermediate results.  All buffers are
+ * caller-owned host memory, borrowed for the duration of the call; nothing
+ * outlives a call.  Every call is synchronous and owns its GPU state
+ * (no module-global SCF state, unlike libint_integrals.f90:15).
+ *
+ * Conventions (the reference's): coordinates in bohr, per-atom interleaved
+ * x y z (Fortran (3, natoms), main.f90:21 / molecule.f90:16); species as
+ * 2*natoms space-padded bytes without NUL (helper_functions.py:105-107);
+ * matrices nbf x nbf row-major (symmetric ones are the same in either order);
+ * basis-function order = the reference's shell order (basis.f90:44-85) x
+ * Cartesian x, y, z (cartesian.f90:7-23).
+ */
+#ifndef PYFACC_H
+#define PYFACC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces run_scf_c (src/c/scf_interface.c:12-17): total energy
+ * E_elec + E_nuc written to *final_energy.  RHF when nalpha == nbeta, else UHF
+ * (the dispatch main.f90:57-61 intends).  Plain Roothaan iterations with the
+ * reference's stopping rule (|dE| < eps and ||dD||_F < eps, rscf.f90:152-155),
+ * at most max_iter iterations, unconverged result returned silently as the
+ * reference does.  Basis file ./data/basis_set/<basis_set_c>.txt relative to
+ * the working directory (main.f90:33), then $PYFACC_DATA/basis_set, then the
+ * data directory shipped next to this library.  On error *final_energy is NaN
+ * and pf_last_error() describes it. */
+void run_scf_c(int natoms, int nelectrons, int charge, int nalpha, int nbeta, const char *species, double *coords,
+               double eps_scf, int max_iter, double *final_energy, const char *basis_set_c, int32_t *z_num);
+
+/* Replaces the Fortran bind(C) entry run_scf_interface_py (main.f90:15-64;
+ * declared at src/c/scf_interface.c:7-9): every argument by reference, the
+ * basis name of length *len_basis_set, not necessarily NUL-terminated. */
+void run_scf_interface_py(int32_t *natoms, int32_t *nelectrons, int32_t *charge, int32_t *nalpha, int32_t *nbeta,
+                          const char *species, double *coords, double *eps_scf, int32_t *max_iter,
+                          double *final_energy, const char *basis_set_c, int32_t *len_basis_set, int32_t *z_num);
+
+/* ---------------------------------------------------------------- pf_* */
+
+enum { PF_OK = 0, PF_ERR_INPUT = 1, PF_ERR_BASIS = 2, PF_ERR_DEVICE = 3, PF_ERR_INTERNAL = 4 };
+enum { PF_PREC_FP64 = 0, PF_PREC_MIXED = 1 };
+enum { PF_ERI_DIRECT = 0, PF_ERI_STORED = 1 };
+enum { PF_DIAG_SP2 = 0, PF_DIAG_JACOBI = 1 };
+
+typedef struct pf_options {
+    double eps;           /* convergence threshold on |dE| and ||dD||_F (rscf.f90:152-155) */
+    int max_iter;
+    int precision;        /* PF_PREC_FP64 or PF_PREC_MIXED (FP32 incremental J/K, FP64 finish) */
+    int incremental;      /* 1: F_n = F_{n-1} + G[D_n - D_{n-1}] (FP64 path) */
+    int eri_mode;         /* PF_ERI_DIRECT or PF_ERI_STORED (FP64 only) */
+    int diis;             /* 0: plain Roothaan (reference); n > 1: Pulay DIIS over n vectors */
+    int diag;             /* PF_DIAG_SP2 (GEMM spectral projector) or PF_DIAG_JACOBI */
+    int renormalize;      /* 1: contracted functions normalised (Libint); 0: primitive norms only (reference) */
+    double schwarz;       /* screening threshold on Q_ab Q_cd max|D| */
+    double mixed_switch;  /* mixed precision: ||dD||_F below which the FP64 finish starts */
+    int verbose;          /* 1: one line per iteration on stderr */
+} pf_options;
+
+typedef struct pf_result {
+    double energy;        /* E_elec + E_nuc (Eh) */
+    double e_elec;
+    double e_nuc;
+    double last_dE;
+    double last_dD;
+    int iterations;
+    int converged;
+    int fock_builds_fp64; /* full or incremental FP64 J/K builds */
+    int fock_builds_fp32; /* FP32 incremental builds (mixed precision) */
+    int nbf;
+    int nshell;
+    double t_total;       /* seconds, wall clock of the SCF call */
+    double t_setup;       /* one-electron integrals, S^-1/2, pair lists, Schwarz, stored ERIs */
+    double t_fock;        /* J/K builds */
+    double t_diag;        /* density from Fock (orthogonalise + SP2/Jacobi + back-transform) */
+    double quartets;      /* shell quartets evaluated over all builds */
+    double min_overlap_eig; /* smallest eigenvalue of S (la.f90:40-44 warns below 1e-8) */
+} pf_result;
+
+/* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
+ * renormalised, Schwarz 1e-10, mixed_switch 1e-5. */
+void pf_default_options(pf_options *opt);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *pf_last_error(void);
+
+/* Number of visible GPUs (0 if none). */
+int pf_device_count(void);
+
+/* Basis size and shells.  Outputs may be NULL; shell arrays hold nshell
+ * entries (exp/coef: nshell*8, coefficients include the primitive norm). */
+int pf_basis_info(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                  int renormalize, int *nbf, int *nshell, int *shell_l, int *shell_atom, int *shell_nprim,
+                  double *shell_exp, double *shell_coef);
+
+/* S, T, V (nbf x nbf) on the GPU (overlap.f90:240, kinetic.f90:117, nuclear.f90:202). */
+int pf_one_electron(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                    int renormalize, double *S, double *T, double *V);
+
+/* All unique ERIs in the canonical packed order of rscf.f90:67-88 (0-based:
+ * ij = i(i+1)/2 + j, i >= j; index = ij(ij+1)/2 + kl, ij >= kl), unscreened;
+ * packed holds nbf(nbf+1)/2 * (nbf(nbf+1)/2 + 1)/2 doubles. */
+int pf_eri_packed(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                  int renormalize, double *packed);
+
+/* J[D] (summed over densities) and K[D_k] per density, for ndens = 1 or 2
+ * host densities (ndens x nbf x nbf).  eri_mode as in pf_options; precision
+ * PF_PREC_MIXED evaluates in FP32.  schwarz <= 0 selects 1e-14. */
+int pf_jk(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+          int renormalize, int ndens, const double *D, double *J, double *K, int eri_mode, int precision,
+          double schwarz);
+
+/* SCF with options.  D_out (ndens x nbf^2; ndens = 1 RHF total density, 2 UHF
+ * alpha, beta) and eps_out (ndens x nbf orbital energies, Jacobi at the end)
+ * may be NULL. */
+int pf_scf(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+           int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *D_out, double *eps_out);
+
+/* Fock-build profile of the last pf_scf call on this thread, summed over its
+ * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,
+ * 1 FP32; classes (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)):
+ * quartets that passed screening, kernel milliseconds (HIP events on the
+ * engine's stream) and kernel launches.  Any output may be NULL. */
+int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches);
+
+/* Device kernels exposed for testing on host buffers. */
+int pf_boys(int nmax, int npts, const double *x, double *F);               /* F[npts*(nmax+1)], nmax <= 6 */
+int pf_eigh(int n, const double *A, double *w, double *V);                  /* Jacobi: w ascending, V columns */
+int pf_projector(int n, const double *F, double nocc, double *P);           /* SP2 spectral projector */
+int pf_gemm(int m, int n, int k, int ta, int tb, double alpha, const double *A, const double *B, double beta,
+            double *C);                                                     /* row-major, FP64 MFMA */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYFACC_H */

@@ -1,0 +1,285 @@
+"""CPU FP64 oracle for the Hartree-Fock SCF path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package, and only as the checker.  The product
+library never calls it.
+
+* ``parse_g94`` restates the reference's basis parser (basis.f90:9-190):
+  element blocks in file order, one shell per atom of that element, SP split
+  into an S shell (all atoms) followed by a P shell (all atoms).
+* The numerics live in ``liboracle.so`` (``oracle.c``): McMurchie-Davidson
+  integrals, exact and trapezoid Boys, J/K, Jacobi eigensolver and the
+  reference's RHF/UHF iteration (rscf.f90:113-157, uscf.f90:140-203).
+
+Parity pins (tests/test_oracle_*.py): closed-form s-type integrals, mpmath
+Boys values, p-type integrals from mpmath derivatives of closed forms with
+respect to the centres, 8-fold ERI symmetry and the Schwarz inequality, J/K
+from the full ERI tensor, and literature HF energies (H2O STO-3G of Crawford's
+project #3, H2 STO-3G of Szabo & Ostlund).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KMAX = 8
+ANG2BOHR = 1.8897259886
+
+_lib = None
+
+
+def build():
+    """Compile ``liboracle.so`` with gcc (oracle/Makefile)."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _lib = ctypes.CDLL(path)
+        _lib.or_boys_trapezoid.restype = ctypes.c_double
+        _lib.or_prim_norm.restype = ctypes.c_double
+        _lib.or_enuc.restype = ctypes.c_double
+        _lib.or_packed_index.restype = ctypes.c_int64
+        _lib.or_jk_direct.restype = ctypes.c_int64
+        _lib.or_scf.argtypes = None
+    return _lib
+
+
+def _p(a, t=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _pi(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+# ---------------------------------------------------------------------------
+# Basis parsing (restates basis.f90:9-190)
+# ---------------------------------------------------------------------------
+def parse_g94(path, species):
+    """Return the list of shells for ``species`` (list of symbols).
+
+    Each shell is ``dict(l, atom, exps, coefs)``; the order follows
+    basis.f90: element blocks in file order; within a block each shell line
+    creates one shell per atom of that element (atom order); an SP line
+    creates the S shells of all those atoms, then their P shells
+    (basis.f90:133-190).  Fortran ``D`` exponents are accepted.
+    """
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    shells = []
+    i = 0
+    n = len(lines)
+    while i < n:
+        line = lines[i]
+        i += 1
+        if not line.strip() or line.startswith("!"):
+            continue
+        tok = line.split()
+        if tok[0] == "****":
+            continue
+        label = tok[0]
+        atoms = [k for k, s in enumerate(species) if s == label]
+        if not atoms:
+            continue
+        while i < n:
+            line = lines[i]
+            i += 1
+            if "****" in line:
+                break
+            tok = line.split()
+            kind, nprim = tok[0], int(tok[1])
+            rows = []
+            for _ in range(nprim):
+                rows.append([float(x.replace("D", "E").replace("d", "e")) for x in lines[i].split()])
+                i += 1
+            exps = [r[0] for r in rows]
+            if kind == "S":
+                shells += [dict(l=0, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
+            elif kind == "P":
+                shells += [dict(l=1, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
+            elif kind == "SP":
+                shells += [dict(l=0, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
+                shells += [dict(l=1, atom=a, exps=exps, coefs=[r[2] for r in rows]) for a in atoms]
+            else:
+                raise ValueError(f"unsupported shell type {kind!r} for {label}")
+    return shells
+
+
+class Basis:
+    """Shell arrays in the oracle's layout (coefficients include the norm)."""
+
+    def __init__(self, shells, coords_bohr, renorm=True):
+        self.nsh = len(shells)
+        self.l = np.array([s["l"] for s in shells], dtype=np.int32)
+        self.np_ = np.array([len(s["exps"]) for s in shells], dtype=np.int32)
+        xyz = np.asarray(coords_bohr, dtype=np.float64)
+        self.ctr = np.ascontiguousarray(np.array([xyz[s["atom"]] for s in shells], dtype=np.float64).reshape(-1))
+        self.alpha = np.zeros(self.nsh * KMAX)
+        raw = np.zeros(self.nsh * KMAX)
+        for k, s in enumerate(shells):
+            self.alpha[k * KMAX:k * KMAX + len(s["exps"])] = s["exps"]
+            raw[k * KMAX:k * KMAX + len(s["exps"])] = s["coefs"]
+        self.coef = np.zeros(self.nsh * KMAX)
+        lib().or_normalize(ctypes.c_int(self.nsh), _pi(self.l), _pi(self.np_), _p(self.alpha), _p(raw),
+                           _p(self.coef), ctypes.c_int(1 if renorm else 0))
+        self.nbf = int(sum((l + 1) * (l + 2) // 2 for l in self.l))
+        self.atom = np.array([s["atom"] for s in shells], dtype=np.int32)
+
+    def args(self):
+        return (ctypes.c_int(self.nsh), _pi(self.l), _pi(self.np_), _p(self.ctr), _p(self.alpha), _p(self.coef))
+
+
+class Molecule:
+    def __init__(self, species, coords_bohr, basis_name="sto-3g", renorm=True, basis_path=None):
+        self.species = list(species)
+        self.xyz = np.ascontiguousarray(np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3))
+        from_z = {s: i + 1 for i, s in enumerate(
+            "H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr"
+            " Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe".split())}
+        self.Z = np.array([from_z[s] for s in self.species], dtype=np.int32)
+        if basis_path is None:
+            basis_path = os.path.join(os.path.dirname(HERE), "pyfacc-scf_amd", "data", "basis_set",
+                                      basis_name + ".txt")
+        self.shells = parse_g94(basis_path, self.species)
+        self.basis = Basis(self.shells, self.xyz, renorm=renorm)
+        self.nbf = self.basis.nbf
+
+    @classmethod
+    def from_xyz(cls, path, basis_name="sto-3g", **kw):
+        with open(path) as fh:
+            lines = fh.read().splitlines()
+        nat = int(lines[0].split()[0])
+        sp, xyz = [], []
+        for ln in lines[2:2 + nat]:
+            t = ln.split()
+            sp.append(t[0])
+            xyz.append([float(t[1]), float(t[2]), float(t[3])])
+        return cls(sp, np.array(xyz) * ANG2BOHR, basis_name, **kw)
+
+    def nelec(self, charge=0):
+        return int(self.Z.sum()) - charge
+
+    # --- integrals -------------------------------------------------------
+    def one_electron(self):
+        n = self.nbf
+        S, T, V = (np.zeros((n, n)) for _ in range(3))
+        lib().or_one_electron(*self.basis.args(), ctypes.c_int(len(self.Z)), _pi(self.Z), _p(self.xyz),
+                              _p(S), _p(T), _p(V))
+        return S, T, V
+
+    def enuc(self):
+        return lib().or_enuc(ctypes.c_int(len(self.Z)), _pi(self.Z), _p(self.xyz))
+
+    def shell_eri(self, a, b, c, d):
+        nc = lambda l: (l + 1) * (l + 2) // 2
+        L = self.basis.l
+        out = np.zeros(nc(L[a]) * nc(L[b]) * nc(L[c]) * nc(L[d]))
+        lib().or_shell_eri(*self.basis.args(), ctypes.c_int(a), ctypes.c_int(b), ctypes.c_int(c), ctypes.c_int(d),
+                           _p(out))
+        return out.reshape(nc(L[a]), nc(L[b]), nc(L[c]), nc(L[d]))
+
+    def eri_packed(self):
+        n = self.nbf
+        npair = n * (n + 1) // 2
+        out = np.zeros(npair * (npair + 1) // 2)
+        lib().or_eri_packed(*self.basis.args(), _p(out))
+        return out
+
+    def eri_full(self):
+        """Full (n,n,n,n) tensor (small systems only)."""
+        n = self.nbf
+        packed = self.eri_packed()
+        i = np.arange(n)
+        ij = np.maximum(i[:, None], i[None, :])
+        ij = ij * (ij + 1) // 2 + np.minimum(i[:, None], i[None, :])
+        a = np.maximum(ij[:, :, None, None], ij[None, None, :, :])
+        b = np.minimum(ij[:, :, None, None], ij[None, None, :, :])
+        return packed[a * (a + 1) // 2 + b]
+
+    def schwarz(self):
+        Q = np.zeros((self.basis.nsh, self.basis.nsh))
+        lib().or_schwarz(*self.basis.args(), _p(Q))
+        return Q
+
+    def jk_packed(self, packed, D):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        nd = 1 if D.ndim == 2 else D.shape[0]
+        J, K = np.zeros_like(D), np.zeros_like(D)
+        lib().or_jk_packed(ctypes.c_int(self.nbf), _p(packed), ctypes.c_int(nd), _p(D), _p(J), _p(K))
+        return J, K
+
+    def jk_direct(self, D, thresh=0.0, Q=None, stride=1):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        nd = 1 if D.ndim == 2 else D.shape[0]
+        if Q is None:
+            Q = self.schwarz()
+        J, K = np.zeros_like(D), np.zeros_like(D)
+        done = lib().or_jk_direct(*self.basis.args(), _p(Q), ctypes.c_double(thresh), ctypes.c_int(nd), _p(D),
+                                  _p(J), _p(K), ctypes.c_int64(stride))
+        return J, K, done
+
+    def scf(self, nalpha, nbeta, eps=1e-10, max_iter=100, eri_mode=0, schwarz=0.0, diis=0):
+        class _Res(ctypes.Structure):
+            _fields_ = [("energy", ctypes.c_double), ("e_elec", ctypes.c_double), ("e_nuc", ctypes.c_double),
+                        ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("fock_builds", ctypes.c_int)]
+        r = _Res()
+        nd = 1 if nalpha == nbeta else 2
+        D = np.zeros((nd, self.nbf, self.nbf))
+        eps_out = np.zeros((nd, self.nbf))
+        lib().or_scf(*self.basis.args(), ctypes.c_int(len(self.Z)), _pi(self.Z), _p(self.xyz), ctypes.c_int(nalpha),
+                     ctypes.c_int(nbeta), ctypes.c_double(eps), ctypes.c_int(max_iter), ctypes.c_int(eri_mode),
+                     ctypes.c_double(schwarz), ctypes.c_int(diis), ctypes.byref(r), _p(D), _p(eps_out))
+        return dict(energy=r.energy, e_elec=r.e_elec, e_nuc=r.e_nuc, iterations=r.iterations,
+                    converged=bool(r.converged), fock_builds=r.fock_builds, D=D if nd == 2 else D[0],
+                    eps=eps_out)
+
+
+def boys_exact(nmax, x):
+    F = np.zeros(nmax + 1)
+    lib().or_boys_exact(ctypes.c_int(nmax), ctypes.c_double(x), _p(F))
+    return F
+
+
+def boys_trapezoid(n, x):
+    return lib().or_boys_trapezoid(ctypes.c_int(n), ctypes.c_double(x))
+
+
+def set_compat_boys(on):
+    lib().or_set_compat_boys(ctypes.c_int(1 if on else 0))
+
+
+def prim_1e(la, lb, a, b, A, B, Z=(), C=()):
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    S, T, V = (np.zeros(nc(la) * nc(lb)) for _ in range(3))
+    Zc = np.array(Z, dtype=np.int32)
+    Cc = np.ascontiguousarray(np.array(C, dtype=np.float64).reshape(-1))
+    A = np.asarray(A, float)
+    B = np.asarray(B, float)
+    lib().or_prim_1e(ctypes.c_int(la), ctypes.c_int(lb), ctypes.c_double(a), ctypes.c_double(b), _p(A), _p(B),
+                     ctypes.c_int(len(Zc)), _pi(Zc) if len(Zc) else None, _p(Cc) if len(Zc) else None,
+                     _p(S), _p(T), _p(V) if len(Zc) else None)
+    return S.reshape(nc(la), nc(lb)), T.reshape(nc(la), nc(lb)), V.reshape(nc(la), nc(lb))
+
+
+def prim_eri(la, lb, lc, ld, a, b, c, d, A, B, C, D):
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    out = np.zeros(nc(la) * nc(lb) * nc(lc) * nc(ld))
+    arrs = [np.asarray(x, float) for x in (A, B, C, D)]
+    lib().or_prim_eri(ctypes.c_int(la), ctypes.c_int(lb), ctypes.c_int(lc), ctypes.c_int(ld), ctypes.c_double(a),
+                      ctypes.c_double(b), ctypes.c_double(c), ctypes.c_double(d), *[_p(x) for x in arrs], _p(out))
+    return out.reshape(nc(la), nc(lb), nc(lc), nc(ld))
+
+
+def set_threads(n):
+    lib().or_set_threads(ctypes.c_int(n))
+
+
+def max_threads():
+    return lib().or_max_threads()

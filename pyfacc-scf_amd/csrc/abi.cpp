@@ -1,0 +1,280 @@
+// C-ABI entry points of libpyfacc.so (declared in include/pyfacc.h).
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "linalg.h"
+
+namespace pf {
+void boys_device(int nmax, int npts, const double *x, double *F, hipStream_t st);
+
+namespace {
+thread_local std::string g_last_error;
+thread_local std::array<unsigned long long, 12> g_prof_q{};
+thread_local std::array<double, 12> g_prof_ms{};
+thread_local std::array<int, 12> g_prof_n{};
+
+struct Input {
+    Molecule mol;
+    std::vector<std::string> species;
+};
+
+Input make_input(int natoms, const char *species, const int32_t *z, const double *coords) {
+    if (natoms <= 0) throw Error("natoms must be positive");
+    if (!species || !z || !coords) throw Error("null species, z or coords");
+    Input in;
+    in.mol.natoms = natoms;
+    in.mol.Z.assign(z, z + natoms);
+    in.mol.xyz.assign(coords, coords + 3 * (size_t)natoms);
+    for (int a = 0; a < natoms; ++a) {
+        std::string s(species + 2 * a, species + 2 * a + 2);
+        while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+        while (!s.empty() && s.front() == ' ') s.erase(s.begin());
+        if (s.empty()) throw Error("empty species label for atom " + std::to_string(a));
+        in.species.push_back(s);
+        if (in.mol.Z[a] <= 0) throw Error("non-positive atomic number for atom " + std::to_string(a));
+    }
+    for (double c : in.mol.xyz)
+        if (!std::isfinite(c)) throw Error("non-finite coordinate");
+    return in;
+}
+
+template <typename Fn>
+int guarded(Fn &&fn) {
+    try {
+        g_last_error.clear();
+        fn();
+        return PF_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        const std::string w = e.what();
+        if (w.find("basis") != std::string::npos) return PF_ERR_BASIS;
+        if (w.find("HIP") != std::string::npos || w.find("GPU") != std::string::npos) return PF_ERR_DEVICE;
+        return PF_ERR_INPUT;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return PF_ERR_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return PF_ERR_INTERNAL;
+    }
+}
+
+}  // namespace
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" {
+
+void pf_default_options(pf_options *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->eps = 1e-8;
+    o->max_iter = 100;
+    o->precision = PF_PREC_FP64;
+    o->incremental = 0;
+    o->eri_mode = PF_ERI_DIRECT;
+    o->diis = 8;
+    o->diag = PF_DIAG_SP2;
+    o->renormalize = 1;
+    o->schwarz = 1e-10;
+    o->mixed_switch = 1e-5;
+    o->verbose = 0;
+}
+
+const char *pf_last_error(void) { return g_last_error.c_str(); }
+
+int pf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pf_basis_info(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                  int renormalize, int *nbf, int *nshell, int *shell_l, int *shell_atom, int *shell_nprim,
+                  double *shell_exp, double *shell_coef) {
+    return guarded([&] {
+        Input in = make_input(natoms, species, z, coords);
+        Basis b = build_basis(resolve_basis_path(basis_name ? basis_name : ""), in.species, in.mol, renormalize != 0);
+        if (nbf) *nbf = b.nbf;
+        if (nshell) *nshell = (int)b.shells.size();
+        for (size_t s = 0; s < b.shells.size(); ++s) {
+            if (shell_l) shell_l[s] = b.shells[s].l;
+            if (shell_atom) shell_atom[s] = b.shells[s].atom;
+            if (shell_nprim) shell_nprim[s] = b.shells[s].np;
+            for (int k = 0; k < KMAX; ++k) {
+                if (shell_exp) shell_exp[s * KMAX + k] = k < b.shells[s].np ? b.shells[s].exp[k] : 0.0;
+                if (shell_coef) shell_coef[s * KMAX + k] = k < b.shells[s].np ? b.shells[s].coef[k] : 0.0;
+            }
+        }
+    });
+}
+
+int pf_one_electron(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                    int renormalize, double *S, double *T, double *V) {
+    return guarded([&] {
+        Input in = make_input(natoms, species, z, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), renormalize != 0, 1e-14);
+        eng.one_electron(S, T, V);
+    });
+}
+
+int pf_eri_packed(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                  int renormalize, double *packed) {
+    return guarded([&] {
+        Input in = make_input(natoms, species, z, coords);
+        // no screening: every unique quartet is written
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), renormalize != 0, 0.0);
+        eng.eri_packed_host(packed);
+    });
+}
+
+int pf_jk(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+          int renormalize, int ndens, const double *D, double *J, double *K, int eri_mode, int precision,
+          double schwarz) {
+    return guarded([&] {
+        if (ndens != 1 && ndens != 2) throw Error("ndens must be 1 or 2");
+        Input in = make_input(natoms, species, z, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), renormalize != 0,
+                   schwarz > 0 ? schwarz : 1e-14);
+        eng.jk_host(ndens, D, J, K, eri_mode, precision);
+    });
+}
+
+int pf_scf(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+           int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *D_out, double *eps_out) {
+    return guarded([&] {
+        pf_options o;
+        if (opt) o = *opt;
+        else pf_default_options(&o);
+        if (o.max_iter < 1) throw Error("max_iter must be >= 1");
+        Input in = make_input(natoms, species, z, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,
+                   o.schwarz > 0 ? o.schwarz : 1e-14);
+        eng.profile_classes = true;
+        pf_result r;
+        eng.scf(nalpha, nbeta, o, r, D_out, eps_out);
+        g_prof_q = eng.prof_quartets;
+        g_prof_ms = eng.prof_ms;
+        g_prof_n = eng.prof_launches;
+        if (res) *res = r;
+    });
+}
+
+int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches) {
+    for (int q = 0; q < 12; ++q) {
+        if (quartets) quartets[q] = g_prof_q[q];
+        if (ms) ms[q] = g_prof_ms[q];
+        if (launches) launches[q] = g_prof_n[q];
+    }
+    return PF_OK;
+}
+
+int pf_boys(int nmax, int npts, const double *x, double *F) {
+    return guarded([&] {
+        if (pf_device_count() == 0) throw Error("no GPU device visible");
+        hipStream_t st;
+        PF_HIP(hipStreamCreate(&st));
+        boys_device(nmax, npts, x, F, st);
+        PF_HIP(hipStreamDestroy(st));
+    });
+}
+
+int pf_eigh(int n, const double *A, double *w, double *V) {
+    return guarded([&] {
+        if (n <= 0) throw Error("n must be positive");
+        if (pf_device_count() == 0) throw Error("no GPU device visible");
+        hipStream_t st;
+        PF_HIP(hipStreamCreate(&st));
+        {
+            DevBuf<double> dA((size_t)n * n), dw(n), dV((size_t)n * n);
+            dA.upload(A, (size_t)n * n, st);
+            jacobi_eigh(n, dA.p, dw.p, dV.p, st);
+            dw.download(w, n, st);
+            dV.download(V, (size_t)n * n, st);
+            PF_HIP(hipStreamSynchronize(st));
+        }
+        PF_HIP(hipStreamDestroy(st));
+    });
+}
+
+int pf_projector(int n, const double *F, double nocc, double *P) {
+    return guarded([&] {
+        if (n <= 0) throw Error("n must be positive");
+        if (pf_device_count() == 0) throw Error("no GPU device visible");
+        hipStream_t st;
+        PF_HIP(hipStreamCreate(&st));
+        {
+            DevBuf<double> dF((size_t)n * n), dP((size_t)n * n), w((size_t)n * n), state(8);
+            dF.upload(F, (size_t)n * n, st);
+            sp2_density(n, dF.p, nocc, dP.p, w.p, state.p, st);
+            dP.download(P, (size_t)n * n, st);
+            PF_HIP(hipStreamSynchronize(st));
+        }
+        PF_HIP(hipStreamDestroy(st));
+    });
+}
+
+int pf_gemm(int m, int n, int k, int ta, int tb, double alpha, const double *A, const double *B, double beta,
+            double *C) {
+    return guarded([&] {
+        if (m <= 0 || n <= 0 || k <= 0) throw Error("gemm sizes must be positive");
+        if (pf_device_count() == 0) throw Error("no GPU device visible");
+        hipStream_t st;
+        PF_HIP(hipStreamCreate(&st));
+        {
+            DevBuf<double> dA((size_t)m * k), dB((size_t)k * n), dC((size_t)m * n);
+            dA.upload(A, (size_t)m * k, st);
+            dB.upload(B, (size_t)k * n, st);
+            dC.upload(C, (size_t)m * n, st);
+            gemm_f64_mnk(m, n, k, ta != 0, tb != 0, alpha, dA.p, ta ? m : k, dB.p, tb ? k : n, beta, dC.p, n, st);
+            dC.download(C, (size_t)m * n, st);
+            PF_HIP(hipStreamSynchronize(st));
+        }
+        PF_HIP(hipStreamDestroy(st));
+    });
+}
+
+void run_scf_c(int natoms, int nelectrons, int charge, int nalpha, int nbeta, const char *species, double *coords,
+               double eps_scf, int max_iter, double *final_energy, const char *basis_set_c, int32_t *z_num) {
+    (void)charge;
+    const int rc = guarded([&] {
+        if (nelectrons != nalpha + nbeta) throw Error("nelectrons != nalpha + nbeta");
+        pf_options o;
+        pf_default_options(&o);
+        o.eps = eps_scf;
+        o.max_iter = max_iter;
+        o.diis = 0;          // plain Roothaan iterations as rscf.f90:118-157
+        o.schwarz = 1e-14;   // the reference keeps every integral
+        Input in = make_input(natoms, species, z_num, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_set_c ? basis_set_c : ""), true, o.schwarz);
+        pf_result r;
+        eng.scf(nalpha, nbeta, o, r, nullptr, nullptr);
+        if (final_energy) *final_energy = r.energy;
+    });
+    if (rc != PF_OK && final_energy) *final_energy = std::numeric_limits<double>::quiet_NaN();
+}
+
+void run_scf_interface_py(int32_t *natoms, int32_t *nelectrons, int32_t *charge, int32_t *nalpha, int32_t *nbeta,
+                          const char *species, double *coords, double *eps_scf, int32_t *max_iter,
+                          double *final_energy, const char *basis_set_c, int32_t *len_basis_set, int32_t *z_num) {
+    if (!natoms || !nelectrons || !charge || !nalpha || !nbeta || !eps_scf || !max_iter || !len_basis_set ||
+        !basis_set_c) {
+        g_last_error = "null argument";
+        if (final_energy) *final_energy = std::numeric_limits<double>::quiet_NaN();
+        return;
+    }
+    const std::string name(basis_set_c, basis_set_c + std::max(0, *len_basis_set));
+    run_scf_c(*natoms, *nelectrons, *charge, *nalpha, *nbeta, species, coords, *eps_scf, *max_iter, final_energy,
+              name.c_str(), z_num);
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// Gaussian-94 basis reader and shell construction.
+//
+// Semantics follow the reference's build_basis (basis.f90:9-92,
+// read_shell :95-130, read_sp_shell :133-190):
+//   * element blocks are visited in file order; a block whose label matches
+//     no atom of the molecule is skipped;
+//   * every shell line of a block creates one shell per atom of that
+//     element, atoms in input order;
+//   * an SP line creates the S shells of all those atoms, then their P shells,
+//     sharing exponents;
+//   * the scale factor on the shell line is read and ignored (basis.f90:75).
+// Basis functions follow shell order, Cartesian components x, y, z for p
+// (cartesian.f90:7-23).  Exponents written with a Fortran 'D' are accepted.
+// Unlike the reference, a skipped block is skipped up to its '****', so an
+// "S"/"P" shell line of a foreign element is never mistaken for a sulfur or
+// phosphorus block header; shells of angular momentum above p are rejected
+// with an error instead of being misread.
+#include <dlfcn.h>
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
+#include "pf_common.h"
+
+namespace pf {
+
+namespace {
+
+bool file_exists(const std::string &p) {
+    struct stat st;
+    return stat(p.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
+std::string lib_dir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&resolve_basis_path), &info) && info.dli_fname) {
+        std::string f = info.dli_fname;
+        auto pos = f.find_last_of('/');
+        if (pos != std::string::npos) return f.substr(0, pos);
+    }
+    return ".";
+}
+
+double parse_real(std::string s) {
+    for (auto &c : s)
+        if (c == 'D' || c == 'd') c = 'E';
+    char *end = nullptr;
+    double v = std::strtod(s.c_str(), &end);
+    if (end == s.c_str()) throw Error("basis file: cannot parse number '" + s + "'");
+    return v;
+}
+
+std::vector<std::string> split(const std::string &line) {
+    std::istringstream is(line);
+    std::vector<std::string> out;
+    std::string t;
+    while (is >> t) out.push_back(t);
+    return out;
+}
+
+std::string trim(const std::string &s) {
+    size_t b = s.find_first_not_of(" \t\r\n"), e = s.find_last_not_of(" \t\r\n");
+    return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
+}
+
+double dfact(int n) {
+    double r = 1.0;
+    for (int k = n; k > 1; k -= 2) r *= k;
+    return r;
+}
+
+// Primitive norm of the (l,0,0) Cartesian component (gaussian.f90:80-98).
+double prim_norm(double a, int l) {
+    return std::pow(2.0 * a / M_PI, 0.75) * std::pow(4.0 * a, 0.5 * l) / std::sqrt(dfact(2 * l - 1));
+}
+
+}  // namespace
+
+std::string resolve_basis_path(const std::string &name) {
+    std::vector<std::string> cands;
+    cands.push_back("./data/basis_set/" + name + ".txt");  // main.f90:33 (CWD-relative)
+    if (const char *env = std::getenv("PYFACC_DATA")) cands.push_back(std::string(env) + "/basis_set/" + name + ".txt");
+    cands.push_back(lib_dir() + "/../data/basis_set/" + name + ".txt");
+    cands.push_back(name);
+    for (auto &c : cands)
+        if (file_exists(c)) return c;
+    std::string msg = "basis set '" + name + "' not found; tried:";
+    for (auto &c : cands) msg += " " + c;
+    throw Error(msg);
+}
+
+Basis build_basis(const std::string &path, const std::vector<std::string> &species, const Molecule &mol,
+                  bool renormalise) {
+    std::ifstream in(path);
+    if (!in) throw Error("cannot open basis file " + path);
+    std::vector<std::string> lines;
+    for (std::string l; std::getline(in, l);) lines.push_back(l);
+
+    Basis basis;
+    size_t i = 0;
+    while (i < lines.size()) {
+        std::string line = lines[i++];
+        if (trim(line).empty() || line[0] == '!') continue;
+        auto tok = split(line);
+        if (tok.empty() || tok[0] == "****") continue;
+        const std::string label = tok[0];
+        std::vector<int> atoms;
+        for (int a = 0; a < (int)species.size(); ++a)
+            if (species[a] == label) atoms.push_back(a);
+        // walk the block to its terminator either way
+        while (i < lines.size()) {
+            std::string sl = lines[i++];
+            if (sl.find("****") != std::string::npos) break;
+            auto st = split(sl);
+            if (st.size() < 2) continue;
+            const std::string kind = st[0];
+            int np = std::atoi(st[1].c_str());
+            if (np <= 0 || np > KMAX) throw Error("basis file: bad primitive count on line '" + sl + "'");
+            std::vector<std::vector<double>> rows;
+            for (int k = 0; k < np; ++k) {
+                if (i >= lines.size()) throw Error("basis file: truncated shell for " + label);
+                std::vector<double> r;
+                for (auto &t : split(lines[i++])) r.push_back(parse_real(t));
+                rows.push_back(r);
+            }
+            if (atoms.empty()) continue;
+            auto add = [&](int l, int col) {
+                for (int a : atoms) {
+                    Shell s;
+                    s.l = l;
+                    s.np = np;
+                    s.atom = a;
+                    for (int d = 0; d < 3; ++d) s.xyz[d] = mol.xyz[3 * a + d];
+                    for (int k = 0; k < np; ++k) {
+                        if ((int)rows[k].size() <= col) throw Error("basis file: missing coefficient for " + label);
+                        s.exp[k] = rows[k][0];
+                        s.coef[k] = rows[k][col];
+                    }
+                    basis.shells.push_back(s);
+                }
+            };
+            if (kind == "S") add(0, 1);
+            else if (kind == "P") add(1, 1);
+            else if (kind == "SP") { add(0, 1); add(1, 2); }
+            else if (!atoms.empty())
+                throw Error("basis file: shell type '" + kind + "' of " + label +
+                            " is not supported (s and p shells only)");
+        }
+    }
+    // every atom must have received functions
+    std::vector<int> seen(species.size(), 0);
+    for (auto &s : basis.shells) seen[s.atom] = 1;
+    for (size_t a = 0; a < species.size(); ++a)
+        if (!seen[a]) throw Error("basis file " + path + " has no functions for element '" + species[a] + "'");
+
+    int bf = 0;
+    for (auto &s : basis.shells) {
+        s.bf = bf;
+        bf += ncart(s.l);
+        for (int k = 0; k < s.np; ++k) s.coef[k] *= prim_norm(s.exp[k], s.l);
+        if (renormalise) {
+            double ss = 0.0;
+            for (int p = 0; p < s.np; ++p)
+                for (int q = 0; q < s.np; ++q) {
+                    double z = s.exp[p] + s.exp[q];
+                    ss += s.coef[p] * s.coef[q] * std::pow(M_PI / z, 1.5) * dfact(2 * s.l - 1) / std::pow(2.0 * z, s.l);
+                }
+            double sc = 1.0 / std::sqrt(ss);
+            for (int k = 0; k < s.np; ++k) s.coef[k] *= sc;
+        }
+    }
+    basis.nbf = bf;
+    return basis;
+}
+
+// E_nuc = sum_{i<j} Z_i Z_j / R_ij, pairs closer than 1e-10 bohr skipped
+// (molecule.f90:35-49).
+double nuclear_repulsion(const Molecule &mol) {
+    double e = 0.0;
+    for (int i = 0; i < mol.natoms; ++i)
+        for (int j = i + 1; j < mol.natoms; ++j) {
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double d = mol.xyz[3 * i + k] - mol.xyz[3 * j + k];
+                d2 += d * d;
+            }
+            double r = std::sqrt(d2);
+            if (r > 1e-10) e += double(mol.Z[i]) * mol.Z[j] / r;
+        }
+    return e;
+}
+
+}  // namespace pf

@@ -1,0 +1,110 @@
+// SCF engine: one molecule on one GPU.
+#pragma once
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "pf_common.h"
+
+extern "C" {
+#include "../../include/pyfacc.h"
+}
+
+namespace pf {
+
+// One shell-pair class on the device, in both precisions.
+struct PairSet {
+    int la = 0, lb = 0;
+    int n = 0, npad = 0, kp = 0;
+    std::vector<int> sa, sb;   // host copies, sorted by Q descending
+    std::vector<double> Q;
+    DevBuf<int> d_sa, d_sb, d_oa, d_ob;
+    DevBuf<double> d_Q, d_AB, d_prim;
+    DevBuf<float> f_Q, f_AB, f_prim;
+    PairView<double> view64() const;
+    PairView<float> view32() const;
+};
+
+struct QuartetClass {
+    int bra, ket;  // indices into Engine::pairs_
+};
+
+struct Timing {
+    double ms = 0.0;
+    int calls = 0;
+};
+
+class Engine {
+   public:
+    Engine(const Molecule &mol, const std::vector<std::string> &species, const std::string &basis_path,
+           bool renormalise, double schwarz);
+    ~Engine();
+
+    int nbf() const { return basis_.nbf; }
+    int nshell() const { return (int)basis_.shells.size(); }
+    const Basis &basis() const { return basis_; }
+    hipStream_t stream() const { return st_; }
+
+    void one_electron(double *S, double *T, double *V);  // host outputs
+    // J/K over host densities (ndens = 1 or 2); host outputs J (total) and K per density.
+    void jk_host(int ndens, const double *D, double *J, double *K, int mode, int precision);
+    void eri_packed_host(double *packed);
+    long long schwarz_pairs() const;
+
+    // Full SCF.  D_out (ndens x nbf^2) and eps_out (ndens x nbf) may be null.
+    void scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out);
+
+    // Fock-build statistics accumulated over an SCF, index precision*6 + class
+    // (precision 0 = FP64, 1 = FP32): quartets, kernel milliseconds (HIP events
+    // on the engine stream, filled when profile_classes is set) and launches.
+    std::array<unsigned long long, 12> prof_quartets{};
+    std::array<double, 12> prof_ms{};
+    std::array<int, 12> prof_launches{};
+    bool profile_classes = false;
+
+   private:
+    void setup_one_electron();
+    void setup_pairs();
+    void setup_store();
+    // G (FP64) += or = G[D] in precision R.  D0/D1 device FP64 densities (D1 null for RHF).
+    template <typename R>
+    unsigned long long build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
+                               int mode);
+    void density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
+                           double *eps_dev);
+
+    Molecule mol_;
+    Basis basis_;
+    double schwarz_;
+    hipStream_t st_ = nullptr;
+    DevBuf<Shell> d_shells_;
+    DevBuf<double> d_axyz_;
+    DevBuf<int> d_Z_;
+    DevBuf<double> d_tab_;
+    DevBuf<float> f_tab_;
+    DevBuf<double> S_, T_, V_, H_, X_;
+    DevBuf<int> d_sh_bf_, d_sh_nf_;
+    std::array<PairSet, 3> pairs_;
+    std::vector<QuartetClass> classes_;
+    bool pairs_ready_ = false;
+    bool one_e_ready_ = false;
+    // J/K accumulators
+    DevBuf<double> J64_, K64a_, K64b_, Dsh64_, dmax64_;
+    DevBuf<float> J32_, K32a_, K32b_, Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;
+    DevBuf<double> Dj64_;
+    DevBuf<unsigned long long> count_;
+    // stored integrals
+    bool store_ready_ = false;
+    std::array<DevBuf<double>, 6> store_;
+    std::array<DevBuf<long long>, 6> store_off_;
+    std::array<DevBuf<int>, 6> store_nket_;
+    // work
+    DevBuf<double> w1_, w2_, w3_, state_;
+};
+
+// Thread-local error text for the C-ABI.
+void set_last_error(const std::string &msg);
+
+}  // namespace pf

@@ -1,0 +1,602 @@
+// SCF engine: integrals, Fock builds and the Roothaan/DIIS loop on one GPU.
+//
+// Control flow follows run_rscf (rscf.f90:49-164) and run_uscf
+// (uscf.f90:65-213): core-Hamiltonian guess, then per iteration
+// G[D] -> F = H + G -> D_old = D -> D from F -> E = 1/2 sum D (H + F)
+// (UHF: 1/2 sum (Da+Db) H + 1/2 sum (Da Fa + Db Fb)) -> stop when
+// |dE| < eps and ||dD||_F < eps.  Everything between the one-time setup and
+// the two scalars read back per iteration stays on the device.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+
+#include "boys.h"
+#include "engine.h"
+#include "linalg.h"
+
+namespace pf {
+
+namespace {
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int round64(int n) { return ((n + 63) / 64) * 64; }
+
+__global__ void scale_cols_kernel(int n, const double *U, const double *s, double *X) {
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) X[(size_t)i * n + j] = U[(size_t)i * n + j] / sqrt(s[j]);
+}
+
+}  // namespace
+
+PairView<double> PairSet::view64() const {
+    PairView<double> v;
+    v.n = n; v.npad = npad; v.kp = kp;
+    v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
+    v.Q = d_Q.p; v.AB = d_AB.p; v.prim = d_prim.p;
+    return v;
+}
+PairView<float> PairSet::view32() const {
+    PairView<float> v;
+    v.n = n; v.npad = npad; v.kp = kp;
+    v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
+    v.Q = f_Q.p; v.AB = f_AB.p; v.prim = f_prim.p;
+    return v;
+}
+
+Engine::Engine(const Molecule &mol, const std::vector<std::string> &species, const std::string &basis_path,
+               bool renormalise, double schwarz)
+    : mol_(mol), schwarz_(schwarz) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error("no GPU device visible");
+    basis_ = build_basis(basis_path, species, mol_, renormalise);
+    PF_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    const int nsh = nshell();
+    d_shells_.upload(basis_.shells.data(), nsh, st_);
+    d_axyz_.upload(mol_.xyz.data(), mol_.xyz.size(), st_);
+    d_Z_.upload(mol_.Z.data(), mol_.Z.size(), st_);
+    const auto tab = boys_table_host();
+    d_tab_.upload(tab.data(), tab.size(), st_);
+    std::vector<float> ftab(tab.begin(), tab.end());
+    f_tab_.upload(ftab.data(), ftab.size(), st_);
+    std::vector<int> bf(nsh), nf(nsh);
+    for (int s = 0; s < nsh; ++s) { bf[s] = basis_.shells[s].bf; nf[s] = ncart(basis_.shells[s].l); }
+    d_sh_bf_.upload(bf.data(), nsh, st_);
+    d_sh_nf_.upload(nf.data(), nsh, st_);
+    PF_HIP(hipStreamSynchronize(st_));
+}
+
+Engine::~Engine() {
+    if (st_) {
+        (void)hipStreamSynchronize(st_);
+        (void)hipStreamDestroy(st_);
+    }
+}
+
+void Engine::setup_one_electron() {
+    if (one_e_ready_) return;
+    const size_t n2 = (size_t)nbf() * nbf();
+    S_.alloc(n2); T_.alloc(n2); V_.alloc(n2); H_.alloc(n2);
+    launch_onee(d_shells_.p, nshell(), nbf(), mol_.natoms, d_axyz_.p, d_Z_.p, d_tab_.p, S_.p, T_.p, V_.p, st_);
+    dev_axpby(n2, 1.0, T_.p, 1.0, V_.p, H_.p, st_);
+    one_e_ready_ = true;
+}
+
+void Engine::one_electron(double *S, double *T, double *V) {
+    setup_one_electron();
+    const size_t n2 = (size_t)nbf() * nbf();
+    if (S) S_.download(S, n2, st_);
+    if (T) T_.download(T, n2, st_);
+    if (V) V_.download(V, n2, st_);
+    PF_HIP(hipStreamSynchronize(st_));
+}
+
+// Shell pairs: class 0 = (s,s), 1 = (p,s), 2 = (p,p); the first shell of a
+// pair has the higher angular momentum (ties: the higher shell index).
+void Engine::setup_pairs() {
+    if (pairs_ready_) return;
+    const int nsh = nshell();
+    const auto &sh = basis_.shells;
+    std::array<std::vector<std::pair<int, int>>, 3> all;
+    for (int a = 0; a < nsh; ++a)
+        for (int b = 0; b <= a; ++b) {
+            int f = a, s = b;
+            if (sh[b].l > sh[a].l) std::swap(f, s);
+            all[sh[f].l + sh[s].l].push_back({f, s});
+        }
+    auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list) {
+        ps.n = (int)list.size();
+        ps.npad = round64(std::max(ps.n, 1));
+        int kp = 1;
+        for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
+        ps.kp = kp;
+        std::vector<int> sa(ps.n), sb(ps.n), oa(ps.n), ob(ps.n);
+        std::vector<double> AB(3 * (size_t)ps.npad, 0.0), prim((size_t)kp * PF_NFIELD * ps.npad, 0.0);
+        for (int u = 0; u < kp; ++u)  // padding entries: unit exponent, zero weight
+            for (int i = 0; i < ps.npad; ++i) prim[((size_t)u * PF_NFIELD + PF_P) * ps.npad + i] = 1.0;
+        const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
+        for (int i = 0; i < ps.n; ++i) {
+            const Shell &A = sh[list[i].first], &B = sh[list[i].second];
+            sa[i] = list[i].first; sb[i] = list[i].second; oa[i] = A.bf; ob[i] = B.bf;
+            double ab[3], r2 = 0.0;
+            for (int d = 0; d < 3; ++d) { ab[d] = A.xyz[d] - B.xyz[d]; r2 += ab[d] * ab[d]; AB[d * ps.npad + i] = ab[d]; }
+            int u = 0;
+            for (int x = 0; x < A.np; ++x)
+                for (int y = 0; y < B.np; ++y, ++u) {
+                    const double a = A.exp[x], b = B.exp[y], p = a + b;
+                    double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
+                    col[PF_P * ps.npad] = p;
+                    for (int d = 0; d < 3; ++d) {
+                        const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
+                        col[(PF_PX + d) * ps.npad] = P;
+                        col[(PF_AX + d) * ps.npad] = P - A.xyz[d];
+                    }
+                    col[PF_K * ps.npad] = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+                }
+            for (; u < kp; ++u) {  // padding primitives contribute zero
+                double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
+                col[PF_P * ps.npad] = 1.0;
+                col[PF_K * ps.npad] = 0.0;
+            }
+        }
+        ps.sa = sa; ps.sb = sb;
+        ps.d_sa.upload(sa.data(), ps.n, st_); ps.d_sb.upload(sb.data(), ps.n, st_);
+        ps.d_oa.upload(oa.data(), ps.n, st_); ps.d_ob.upload(ob.data(), ps.n, st_);
+        ps.d_AB.upload(AB.data(), AB.size(), st_);
+        ps.d_prim.upload(prim.data(), prim.size(), st_);
+        std::vector<float> fAB(AB.begin(), AB.end()), fprim(prim.begin(), prim.end());
+        ps.f_AB.upload(fAB.data(), fAB.size(), st_);
+        ps.f_prim.upload(fprim.data(), fprim.size(), st_);
+    };
+    const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+    // pass 1: all pairs, Schwarz bounds
+    std::array<std::vector<double>, 3> Qall;
+    for (int c = 0; c < 3; ++c) {
+        PairSet &ps = pairs_[c];
+        ps.la = cls_l[c][0]; ps.lb = cls_l[c][1];
+        fill(ps, all[c]);
+        ps.d_Q.alloc(std::max(ps.n, 1));
+        launch_schwarz<double>(ps.la, ps.lb, ps.view64(), d_tab_.p, ps.d_Q.p, st_);
+        Qall[c].resize(ps.n);
+        ps.d_Q.download(Qall[c].data(), ps.n, st_);
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+    double qmax = 0.0;
+    for (auto &q : Qall) for (double v : q) qmax = std::max(qmax, v);
+    // pass 2: keep pairs that can pass Q_ab Q_max >= tau, sorted by Q descending
+    for (int c = 0; c < 3; ++c) {
+        std::vector<int> idx;
+        for (int i = 0; i < (int)Qall[c].size(); ++i)
+            if (Qall[c][i] * qmax >= schwarz_) idx.push_back(i);
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return Qall[c][x] > Qall[c][y]; });
+        std::vector<std::pair<int, int>> kept;
+        std::vector<double> q;
+        for (int i : idx) { kept.push_back(all[c][i]); q.push_back(Qall[c][i]); }
+        PairSet &ps = pairs_[c];
+        fill(ps, kept);
+        ps.Q = q;
+        ps.d_Q.upload(q.data(), q.size(), st_);
+        std::vector<float> fq(q.begin(), q.end());
+        // FP32 bound rounded up so FP32 screening never drops what FP64 keeps
+        for (auto &v : fq) v = std::nextafter(v, 1e30f);
+        ps.f_Q.upload(fq.data(), fq.size(), st_);
+    }
+    classes_.clear();
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y <= x; ++y) classes_.push_back({x, y});
+    const size_t n2 = (size_t)nbf() * nbf();
+    J64_.alloc(n2); K64a_.alloc(n2); K64b_.alloc(n2); Dj64_.alloc(n2);
+    Dsh64_.alloc((size_t)nshell() * nshell()); dmax64_.alloc(1);
+    count_.alloc(6);
+    PF_HIP(hipStreamSynchronize(st_));
+    pairs_ready_ = true;
+}
+
+long long Engine::schwarz_pairs() const {
+    long long n = 0;
+    for (auto &p : pairs_) n += p.n;
+    return n;
+}
+
+void Engine::setup_store() {
+    if (store_ready_) return;
+    setup_pairs();
+    const int ints[3] = {1, 3, 9};
+    for (size_t q = 0; q < classes_.size(); ++q) {
+        const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
+        const bool same = classes_[q].bra == classes_[q].ket;
+        const int nint = ints[classes_[q].bra] * ints[classes_[q].ket];
+        std::vector<long long> off(std::max(B.n, 1), 0);
+        std::vector<int> nk(std::max(B.n, 1), 0);
+        long long tot = 0;
+        for (int i = 0; i < B.n; ++i) {
+            const double thr = schwarz_ / B.Q[i];
+            int lo = 0, hi = K.n;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (K.Q[mid] >= thr) lo = mid + 1; else hi = mid;
+            }
+            nk[i] = same ? std::min(lo, i + 1) : lo;
+            off[i] = tot;
+            tot += (long long)round64(nk[i]) * nint;
+        }
+        store_[q].alloc(std::max<long long>(tot, 1));
+        store_off_[q].upload(off.data(), off.size(), st_);
+        store_nket_[q].upload(nk.data(), nk.size(), st_);
+        JKArgs<double> a;
+        a.bra = B.view64(); a.ket = K.view64(); a.same = same; a.tau = schwarz_;
+        a.nbf = nbf(); a.nsh = nshell(); a.tab = d_tab_.p;
+        a.store = store_[q].p; a.store_off = store_off_[q].p; a.store_nket = store_nket_[q].p;
+        launch_jk_class<double>(pairs_[classes_[q].bra].la, pairs_[classes_[q].bra].lb, pairs_[classes_[q].ket].la,
+                                pairs_[classes_[q].ket].lb, 1, MODE_STORE, a, st_);
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+    store_ready_ = true;
+}
+
+template <typename R>
+unsigned long long Engine::build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
+                                   int mode) {
+    setup_pairs();
+    const int n = nbf(), nsh = nshell();
+    const size_t n2 = (size_t)n * n;
+    const bool uhf = D1 != nullptr;
+    JKArgs<R> a;
+    a.nbf = n; a.nsh = nsh; a.tau = R(schwarz_);
+    R *J, *Ka, *Kb;
+    if constexpr (std::is_same<R, double>::value) {
+        J = J64_.p; Ka = K64a_.p; Kb = K64b_.p;
+        if (uhf) {
+            dev_axpby(n2, 1.0, D0, 1.0, D1, Dj64_.p, st_);
+            a.Dj = Dj64_.p; a.Dk0 = D0; a.Dk1 = D1;
+        } else {
+            a.Dj = D0; a.Dk0 = D0;
+        }
+        a.tab = d_tab_.p;
+        dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, D0, D1, Dsh64_.p, dmax64_.p, st_);
+        a.Dsh = Dsh64_.p; a.dmax = dmax64_.p;
+        dev_fill(n2, 0.0, J, st_); dev_fill(n2, 0.0, Ka, st_);
+        if (uhf) dev_fill(n2, 0.0, Kb, st_);
+    } else {
+        if (!J32_.p) {
+            J32_.alloc(n2); K32a_.alloc(n2); K32b_.alloc(n2); Dj32_.alloc(n2); Dk32a_.alloc(n2); Dk32b_.alloc(n2);
+            Dsh32_.alloc((size_t)nsh * nsh); dmax32_.alloc(1);
+        }
+        J = J32_.p; Ka = K32a_.p; Kb = K32b_.p;
+        if (uhf) {
+            dev_to_float(n2, 1.0, D0, D1, Dj32_.p, st_);
+            dev_to_float(n2, 1.0, D0, nullptr, Dk32a_.p, st_);
+            dev_to_float(n2, 1.0, D1, nullptr, Dk32b_.p, st_);
+            a.Dj = Dj32_.p; a.Dk0 = Dk32a_.p; a.Dk1 = Dk32b_.p;
+        } else {
+            dev_to_float(n2, 1.0, D0, nullptr, Dj32_.p, st_);
+            a.Dj = Dj32_.p; a.Dk0 = Dj32_.p;
+        }
+        a.tab = f_tab_.p;
+        dev_shell_dmax<float>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, D0, D1, Dsh32_.p, dmax32_.p, st_);
+        a.Dsh = Dsh32_.p; a.dmax = dmax32_.p;
+        dev_fillf(n2, 0.f, J, st_); dev_fillf(n2, 0.f, Ka, st_);
+        if (uhf) dev_fillf(n2, 0.f, Kb, st_);
+    }
+    a.J = J; a.K0 = Ka; a.K1 = uhf ? Kb : nullptr;
+    PF_HIP(hipMemsetAsync(count_.p, 0, 6 * sizeof(unsigned long long), st_));
+    hipEvent_t ev[7];
+    if (profile_classes)
+        for (auto &e : ev) PF_HIP(hipEventCreate(&e));
+    for (size_t q = 0; q < classes_.size(); ++q) {
+        const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
+        if constexpr (std::is_same<R, double>::value) { a.bra = B.view64(); a.ket = K.view64(); }
+        else { a.bra = B.view32(); a.ket = K.view32(); }
+        a.same = classes_[q].bra == classes_[q].ket;
+        if (mode == MODE_LOAD) {
+            if constexpr (std::is_same<R, double>::value) {
+                a.store = store_[q].p; a.store_off = store_off_[q].p; a.store_nket = store_nket_[q].p;
+            }
+        }
+        a.count = count_.p + q;
+        if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
+        launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
+    }
+    if (profile_classes) PF_HIP(hipEventRecord(ev[6], st_));
+    dev_assemble_g<R>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
+    unsigned long long cnt6[6] = {0, 0, 0, 0, 0, 0};
+    PF_HIP(hipMemcpyAsync(cnt6, count_.p, sizeof(cnt6), hipMemcpyDeviceToHost, st_));
+    PF_HIP(hipStreamSynchronize(st_));
+    const int pbase = std::is_same<R, double>::value ? 0 : 6;
+    unsigned long long cnt = 0;
+    for (int q = 0; q < 6; ++q) {
+        prof_quartets[pbase + q] += cnt6[q];
+        prof_launches[pbase + q] += 1;
+        cnt += cnt6[q];
+    }
+    if (profile_classes) {
+        for (int q = 0; q < 6; ++q) {
+            float ms = 0.f;
+            PF_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+            prof_ms[pbase + q] += ms;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    return cnt;
+}
+
+void Engine::jk_host(int ndens, const double *D, double *J, double *K, int mode, int precision) {
+    setup_pairs();
+    const int n = nbf();
+    const size_t n2 = (size_t)n * n;
+    DevBuf<double> dD(n2 * ndens), G0(n2), G1(n2);
+    dD.upload(D, n2 * ndens, st_);
+    int kmode = MODE_DIRECT;
+    if (mode == PF_ERI_STORED) {
+        if (precision != PF_PREC_FP64) throw Error("stored ERIs are FP64 only");
+        setup_store();
+        kmode = MODE_LOAD;
+    }
+    // J and K separately: run with accumulate=false, then read the raw accumulators
+    const double *D1 = ndens == 2 ? dD.p + n2 : nullptr;
+    if (precision == PF_PREC_FP64) build_g<double>(dD.p, D1, G0.p, G1.p, false, kmode);
+    else build_g<float>(dD.p, D1, G0.p, G1.p, false, kmode);
+    // symmetrised J', K' -> host
+    std::vector<double> j(n2), k0(n2), k1(ndens == 2 ? n2 : 0);
+    if (precision == PF_PREC_FP64) {
+        J64_.download(j.data(), n2, st_);
+        K64a_.download(k0.data(), n2, st_);
+        if (ndens == 2) K64b_.download(k1.data(), n2, st_);
+    } else {
+        std::vector<float> f(n2);
+        J32_.download(f.data(), n2, st_); PF_HIP(hipStreamSynchronize(st_));
+        std::copy(f.begin(), f.end(), j.begin());
+        K32a_.download(f.data(), n2, st_); PF_HIP(hipStreamSynchronize(st_));
+        std::copy(f.begin(), f.end(), k0.begin());
+        if (ndens == 2) {
+            K32b_.download(f.data(), n2, st_); PF_HIP(hipStreamSynchronize(st_));
+            std::copy(f.begin(), f.end(), k1.begin());
+        }
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) {
+            J[(size_t)r * n + c] = j[(size_t)r * n + c] + j[(size_t)c * n + r];
+            K[(size_t)r * n + c] = k0[(size_t)r * n + c] + k0[(size_t)c * n + r];
+            if (ndens == 2) K[n2 + (size_t)r * n + c] = k1[(size_t)r * n + c] + k1[(size_t)c * n + r];
+        }
+}
+
+void Engine::eri_packed_host(double *packed) {
+    setup_pairs();
+    const long long np = (long long)nbf() * (nbf() + 1) / 2;
+    const long long total = np * (np + 1) / 2;
+    DevBuf<double> d(total);
+    dev_fill(total, 0.0, d.p, st_);
+    for (size_t q = 0; q < classes_.size(); ++q) {
+        const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
+        JKArgs<double> a;
+        a.bra = B.view64(); a.ket = K.view64(); a.same = classes_[q].bra == classes_[q].ket;
+        a.nbf = nbf(); a.nsh = nshell(); a.tab = d_tab_.p; a.packed = d.p;
+        launch_jk_class<double>(B.la, B.lb, K.la, K.lb, 1, MODE_PACKED, a, st_);
+    }
+    d.download(packed, total, st_);
+    PF_HIP(hipStreamSynchronize(st_));
+}
+
+// D = occ * X P X^T with P the projector onto the nocc lowest eigenvectors
+// of F' = X^T F X (la.f90:52-54 and rscf.f90:140 / uscf.f90:183-184).
+void Engine::density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
+                               double *eps_dev) {
+    const int n = nbf();
+    gemm_f64(n, false, false, 1.0, F, X_.p, 0.0, w1_.p, st_);   // F X
+    gemm_f64(n, true, false, 1.0, X_.p, w1_.p, 0.0, w2_.p, st_);  // X^T F X
+    if (nocc == 0) {
+        dev_fill((size_t)n * n, 0.0, D, st_);
+        if (eps_dev) {
+            DevBuf<double> C(size_t(n) * n);
+            jacobi_eigh(n, w2_.p, eps_dev, C.p, st_);
+        }
+        return;
+    }
+    if (opt.diag == PF_DIAG_JACOBI || eps_dev) {
+        DevBuf<double> w(n), C(size_t(n) * n);
+        jacobi_eigh(n, w2_.p, eps_dev ? eps_dev : w.p, C.p, st_);
+        gemm_f64(n, false, false, 1.0, X_.p, C.p, 0.0, w1_.p, st_);  // C = X C'
+        gemm_f64_mnk(n, n, nocc, false, true, occ, w1_.p, n, w1_.p, n, 0.0, D, n, st_);
+        return;
+    }
+    sp2_density(n, w2_.p, double(nocc), w3_.p, w1_.p, state_.p, st_);
+    gemm_f64(n, false, false, 1.0, X_.p, w3_.p, 0.0, w1_.p, st_);       // X P
+    gemm_f64(n, false, true, occ, w1_.p, X_.p, 0.0, D, st_);            // occ X P X^T
+}
+
+void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out) {
+    const double t0 = now();
+    res = pf_result{};
+    const int n = nbf();
+    const size_t n2 = (size_t)n * n;
+    const bool uhf = nalpha != nbeta;
+    const int nd = uhf ? 2 : 1;
+    res.nbf = n;
+    res.nshell = nshell();
+    res.e_nuc = nuclear_repulsion(mol_);
+    if (nalpha < 0 || nbeta < 0 || nalpha > n || nbeta > n)
+        throw Error("electron count does not fit the basis (nalpha=" + std::to_string(nalpha) +
+                    ", nbeta=" + std::to_string(nbeta) + ", nbf=" + std::to_string(n) + ")");
+    if (opt.eri_mode == PF_ERI_STORED && opt.precision != PF_PREC_FP64)
+        throw Error("stored ERIs are FP64 only");
+
+    setup_one_electron();
+    w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); state_.alloc(8);
+    // X = U s^{-1/2} (la.f90:36-50), computed once
+    X_.alloc(n2);
+    {
+        DevBuf<double> s(n), U(n2);
+        jacobi_eigh(n, S_.p, s.p, U.p, st_);
+        std::vector<double> hs(n);
+        s.download(hs.data(), n, st_);
+        PF_HIP(hipStreamSynchronize(st_));
+        res.min_overlap_eig = n ? hs[0] : 0.0;
+        if (n && hs[0] <= 0.0) throw Error("overlap matrix is not positive definite (coincident atoms?)");
+        dim3 g2((n + 127) / 128, n);
+        scale_cols_kernel<<<g2, 128, 0, st_>>>(n, U.p, s.p, X_.p);
+        PF_HIP(hipGetLastError());
+    }
+    setup_pairs();
+    if (opt.eri_mode == PF_ERI_STORED) setup_store();
+    PF_HIP(hipStreamSynchronize(st_));
+    res.t_setup = now() - t0;
+
+    const int nocc[2] = {uhf ? nalpha : nalpha, nbeta};
+    const double occ = uhf ? 1.0 : 2.0;
+    DevBuf<double> D(n2 * nd), Dold(n2 * nd), Dbuilt(n2 * nd), dD(n2 * nd), G(n2 * nd), F(n2 * nd), red(4);
+    // core guess (rscf.f90:113-114; uscf.f90:140-144, C_alpha = C_beta)
+    {
+        const double td = now();
+        for (int s = 0; s < nd; ++s) density_from_fock(H_.p, nocc[s], occ, D.p + s * n2, opt, nullptr);
+        PF_HIP(hipStreamSynchronize(st_));
+        res.t_diag += now() - td;
+    }
+    dev_fill(n2 * nd, 0.0, Dbuilt.p, st_);
+    dev_fill(n2 * nd, 0.0, G.p, st_);
+
+    // DIIS storage
+    const int ndiis = opt.diis > 1 ? std::min(opt.diis, 32) : 0;
+    std::vector<DevBuf<double>> dF(ndiis), dE(ndiis);
+    for (int k = 0; k < ndiis; ++k) { dF[k].alloc(n2 * nd); dE[k].alloc(n2 * nd); }
+    std::vector<double> Bmat((size_t)ndiis * ndiis, 0.0);
+    DevBuf<double> dots(std::max(ndiis, 1)), Fx(ndiis ? n2 * nd : 1);
+    int nstore = 0, head = 0;
+
+    bool fp32_phase = opt.precision == PF_PREC_MIXED;
+    const bool incremental = opt.incremental || opt.precision == PF_PREC_MIXED;
+    const int gmode = opt.eri_mode == PF_ERI_STORED ? MODE_LOAD : MODE_DIRECT;
+    double Eold = 0.0, Eel = 0.0, last_dD = 1e300;
+    int it = 0;
+    bool conv = false;
+    for (it = 1; it <= opt.max_iter; ++it) {
+        // ---- Fock build
+        const double tf = now();
+        const double *Dcur0 = D.p, *Dcur1 = uhf ? D.p + n2 : nullptr;
+        if (fp32_phase && last_dD < opt.mixed_switch) {
+            // leave FP32: one full FP64 build, FP64 increments afterwards
+            fp32_phase = false;
+            res.quartets += build_g<double>(Dcur0, Dcur1, G.p, uhf ? G.p + n2 : nullptr, false, gmode);
+            ++res.fock_builds_fp64;
+            PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+        } else if (incremental) {
+            dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
+            const double *d0 = dD.p, *d1 = uhf ? dD.p + n2 : nullptr;
+            if (fp32_phase) {
+                res.quartets += build_g<float>(d0, d1, G.p, uhf ? G.p + n2 : nullptr, true, MODE_DIRECT);
+                ++res.fock_builds_fp32;
+            } else {
+                res.quartets += build_g<double>(d0, d1, G.p, uhf ? G.p + n2 : nullptr, true, gmode);
+                ++res.fock_builds_fp64;
+            }
+            PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+        } else {
+            res.quartets += build_g<double>(Dcur0, Dcur1, G.p, uhf ? G.p + n2 : nullptr, false, gmode);
+            ++res.fock_builds_fp64;
+        }
+        for (int s = 0; s < nd; ++s) dev_axpby(n2, 1.0, H_.p, 1.0, G.p + s * n2, F.p + s * n2, st_);
+        PF_HIP(hipStreamSynchronize(st_));
+        res.t_fock += now() - tf;
+
+        const double td = now();
+        PF_HIP(hipMemcpyAsync(Dold.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+        const double *Fuse = F.p;
+        if (ndiis) {
+            // error e = F D S - S D F per spin, with the density F was built from
+            const int slot = head;
+            for (int s = 0; s < nd; ++s) {
+                gemm_f64(n, false, false, 1.0, F.p + s * n2, D.p + s * n2, 0.0, w1_.p, st_);
+                gemm_f64(n, false, false, 1.0, w1_.p, S_.p, 0.0, w2_.p, st_);
+                dev_antisym(n, w2_.p, dE[slot].p + s * n2, st_);
+            }
+            PF_HIP(hipMemcpyAsync(dF[slot].p, F.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+            head = (head + 1) % ndiis;
+            if (nstore < ndiis) ++nstore;
+            for (int k = 0; k < nstore; ++k) dev_dot(n2 * nd, dE[slot].p, dE[k].p, dots.p + k, st_);
+            std::vector<double> row(nstore);
+            dots.download(row.data(), nstore, st_);
+            PF_HIP(hipStreamSynchronize(st_));
+            for (int k = 0; k < nstore; ++k) { Bmat[slot * ndiis + k] = row[k]; Bmat[k * ndiis + slot] = row[k]; }
+            if (nstore >= 2) {
+                // solve [B -1; -1 0] [c; l] = [0; -1] (Gaussian elimination with pivoting)
+                const int m = nstore, N = m + 1;
+                std::vector<int> idx;
+                for (int k = 0; k < ndiis && (int)idx.size() < m; ++k)
+                    if (k < nstore) idx.push_back(k);
+                std::vector<double> M((size_t)N * (N + 1), 0.0);
+                for (int r = 0; r < m; ++r) {
+                    for (int c = 0; c < m; ++c) M[r * (N + 1) + c] = Bmat[idx[r] * ndiis + idx[c]];
+                    M[r * (N + 1) + m] = -1.0;
+                }
+                for (int c = 0; c < m; ++c) M[m * (N + 1) + c] = -1.0;
+                M[m * (N + 1) + N] = -1.0;
+                bool ok = true;
+                for (int col = 0; col < N && ok; ++col) {
+                    int piv = col;
+                    for (int r = col + 1; r < N; ++r)
+                        if (std::fabs(M[r * (N + 1) + col]) > std::fabs(M[piv * (N + 1) + col])) piv = r;
+                    if (std::fabs(M[piv * (N + 1) + col]) < 1e-300) { ok = false; break; }
+                    if (piv != col)
+                        for (int k = 0; k <= N; ++k) std::swap(M[col * (N + 1) + k], M[piv * (N + 1) + k]);
+                    for (int r = 0; r < N; ++r) {
+                        if (r == col) continue;
+                        const double f = M[r * (N + 1) + col] / M[col * (N + 1) + col];
+                        for (int k = col; k <= N; ++k) M[r * (N + 1) + k] -= f * M[col * (N + 1) + k];
+                    }
+                }
+                if (ok) {
+                    dev_fill(n2 * nd, 0.0, Fx.p, st_);
+                    for (int r = 0; r < m; ++r) {
+                        const double c = M[r * (N + 1) + N] / M[r * (N + 1) + r];
+                        dev_axpby(n2 * nd, 1.0, Fx.p, c, dF[idx[r]].p, Fx.p, st_);
+                    }
+                    Fuse = Fx.p;
+                }
+            }
+        }
+        for (int s = 0; s < nd; ++s) density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr);
+        // energy with the new D and the F built this iteration (rscf.f90:144-151; uscf.f90:186-193)
+        if (!uhf) dev_energy_rhf(n, D.p, H_.p, F.p, red.p, st_);
+        else dev_energy_uhf(n, D.p, D.p + n2, H_.p, F.p, F.p + n2, red.p, st_);
+        dev_diff_sq(n2 * nd, D.p, Dold.p, red.p + 1, st_);
+        double hv[2];
+        PF_HIP(hipMemcpyAsync(hv, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+        PF_HIP(hipStreamSynchronize(st_));
+        res.t_diag += now() - td;
+        Eel = uhf ? hv[0] : 0.5 * hv[0];
+        const double dE_ = std::fabs(Eel - Eold), errD = std::sqrt(hv[1]);
+        last_dD = errD;
+        res.last_dE = dE_;
+        res.last_dD = errD;
+        if (opt.verbose)
+            std::fprintf(stderr, "iter %3d  E = %.12f  dE = %.3e  dD = %.3e  %s\n", it, Eel + res.e_nuc, dE_, errD,
+                         fp32_phase ? "fp32" : "fp64");
+        if (!std::isfinite(Eel)) throw Error("SCF produced a non-finite energy");
+        if (dE_ < opt.eps && errD < opt.eps && !fp32_phase) { conv = true; break; }
+        Eold = Eel;
+    }
+    res.iterations = std::min(it, opt.max_iter);
+    res.converged = conv ? 1 : 0;
+    res.e_elec = Eel;
+    res.energy = Eel + res.e_nuc;
+    if (D_out) { D.download(D_out, n2 * nd, st_); PF_HIP(hipStreamSynchronize(st_)); }
+    if (eps_out) {
+        DevBuf<double> e(n), Dtmp(n2);
+        for (int s = 0; s < nd; ++s) {
+            density_from_fock(F.p + s * n2, nocc[s], occ, Dtmp.p, opt, e.p);
+            e.download(eps_out + (size_t)s * n, n, st_);
+        }
+        PF_HIP(hipStreamSynchronize(st_));
+    }
+    res.t_total = now() - t0;
+}
+
+template unsigned long long Engine::build_g<double>(const double *, const double *, double *, double *, bool, int);
+template unsigned long long Engine::build_g<float>(const double *, const double *, double *, double *, bool, int);
+
+}  // namespace pf

@@ -1,0 +1,74 @@
+// Internal kernel launchers and device views shared by the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pf_common.h"
+
+namespace pf {
+
+// ---------------------------------------------------------------- 1e
+void launch_onee(const Shell *d_shells, int nsh, int nbf, int natoms, const double *d_axyz, const int *d_Z,
+                 const double *d_tab, double *S, double *T, double *V, hipStream_t st);
+
+// ---------------------------------------------------------------- pairs
+// Per primitive pair data, in this field order, stored structure-of-arrays.
+enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_NFIELD };
+// PF_AX.. hold P - A (A = first shell of the pair); PF_K holds
+// sqrt(2) pi^{5/4} c_a c_b exp(-ab/p |AB|^2) / p, so that the quartet
+// prefactor is K_bra K_ket / sqrt(p + q).
+
+// Device view of one shell-pair class (LA >= LB), pairs sorted by Schwarz
+// bound Q descending.  Arrays are structure-of-arrays with stride npad.
+template <typename R>
+struct PairView {
+    int n = 0, npad = 0, kp = 0;
+    const int *sa = nullptr, *sb = nullptr;  // shell indices
+    const int *oa = nullptr, *ob = nullptr;  // first basis function of each shell
+    const R *Q = nullptr;                    // [n] Schwarz bound
+    const R *AB = nullptr;                   // [3][npad]  A - B
+    const R *prim = nullptr;                 // [kp][PF_NFIELD][npad]
+};
+
+// Work description shared by all ERI/J/K kernels of one launch.
+template <typename R>
+struct JKArgs {
+    PairView<R> bra, ket;
+    int same = 0;          // bra and ket are the same class: only ket j <= bra i
+    R tau = 0;             // screening threshold on Q_i Q_j dmax
+    int nbf = 0, nsh = 0;
+    const R *Dj = nullptr;   // density for J (total density)
+    const R *Dk0 = nullptr;  // densities for K
+    const R *Dk1 = nullptr;
+    const R *Dsh = nullptr;  // [nsh*nsh] max |D| per shell block (screening), nullptr = Schwarz only
+    const R *dmax = nullptr; // device scalar: 2 * max(Dsh)
+    R *J = nullptr, *K0 = nullptr, *K1 = nullptr;  // non-symmetric accumulators J', K'
+    const R *tab = nullptr;  // Boys table, stride BOYS_MMAX+1
+    // stored-integral mode
+    R *store = nullptr;
+    const long long *store_off = nullptr;  // per bra: first element in store
+    const int *store_nket = nullptr;       // per bra: number of stored kets
+    // packed canonical export
+    double *packed = nullptr;
+    // statistics (quartets that passed screening), may be nullptr
+    unsigned long long *count = nullptr;
+};
+
+enum JKMode { MODE_DIRECT = 0, MODE_STORE = 1, MODE_LOAD = 2, MODE_PACKED = 3 };
+
+// Launch one quartet class (bra class la,lb | ket class lc,ld).
+template <typename R>
+void launch_jk_class(int la, int lb, int lc, int ld, int nk, int mode, const JKArgs<R> &args, hipStream_t st);
+
+// Schwarz bounds for every pair of a class: Q[i] = sqrt(max |(ab|ab)|).
+template <typename R>
+void launch_schwarz(int la, int lb, const PairView<R> &pv, const R *tab, R *Q, hipStream_t st);
+
+// ---------------------------------------------------------------- dense FP64
+// C = alpha op(A) op(B) + beta C, row-major n x n (lda = ldb = ldc = n), MFMA f64.
+void gemm_f64(int n, bool ta, bool tb, double alpha, const double *A, const double *B, double beta, double *C,
+              hipStream_t st);
+// General m x n x k row-major variant.
+void gemm_f64_mnk(int m, int n, int k, bool ta, bool tb, double alpha, const double *A, int lda,
+                  const double *B, int ldb, double beta, double *C, int ldc, hipStream_t st);
+
+}  // namespace pf

@@ -1,0 +1,103 @@
+// Shared host/device definitions for the MI355X Hartree-Fock engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace pf {
+
+constexpr int KMAX = 8;       // max primitives per shell (STO-6G needs 6)
+constexpr int LMAX = 1;       // s and p shells (basis.f90:76-82 handles S, P, SP only)
+constexpr int BOYS_MMAX = 12; // highest Boys order held in the table
+constexpr int BOYS_TAYLOR = 7;
+constexpr double BOYS_H = 1.0 / 16.0;  // table spacing in x
+constexpr double BOYS_XMAX = 40.0;     // beyond this the asymptotic form is used
+constexpr int BOYS_NGRID = 642;        // x = 0 .. XMAX + 2h
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define PF_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::pf::Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                              __FILE__ + ":" + std::to_string(__LINE__));                \
+    } while (0)
+
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// One contracted shell.  coef[] already contains the primitive norm of the
+// (l,0,0) component (gaussian.f90:80-98) and, by default, the contraction
+// renormalisation (Libint convention).
+struct Shell {
+    int l = 0;
+    int np = 0;
+    int atom = 0;
+    int bf = 0;  // first basis function
+    double xyz[3] = {0, 0, 0};
+    double exp[KMAX] = {0};
+    double coef[KMAX] = {0};
+};
+
+struct Molecule {
+    int natoms = 0;
+    std::vector<int> Z;
+    std::vector<double> xyz;  // 3*natoms, bohr, per-atom interleaved
+};
+
+struct Basis {
+    std::vector<Shell> shells;
+    int nbf = 0;
+};
+
+// Basis construction (basis.cpp).
+Basis build_basis(const std::string &path, const std::vector<std::string> &species, const Molecule &mol,
+                  bool renormalise);
+std::string resolve_basis_path(const std::string &name);
+double nuclear_repulsion(const Molecule &mol);
+
+// Device buffer with RAII.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) PF_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        release();
+        p = o.p; n = o.n; o.p = nullptr; o.n = 0;
+        return *this;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        // synchronous: the host source may be a temporary
+        if (count) {
+            PF_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+            PF_HIP(hipStreamSynchronize(s));
+        }
+    }
+    void download(T *h, size_t count, hipStream_t s) const {
+        if (count) PF_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+}  // namespace pf

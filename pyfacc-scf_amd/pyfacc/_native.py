@@ -1,0 +1,231 @@
+"""ctypes binding of libpyfacc.so (declarations in include/pyfacc.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("PYFACC_LIB", os.path.join(_HERE, "lib", "libpyfacc.so"))
+
+PF_PREC_FP64, PF_PREC_MIXED = 0, 1
+PF_ERI_DIRECT, PF_ERI_STORED = 0, 1
+PF_DIAG_SP2, PF_DIAG_JACOBI = 0, 1
+
+
+class PyfaccError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_double), ("max_iter", ctypes.c_int), ("precision", ctypes.c_int),
+                ("incremental", ctypes.c_int), ("eri_mode", ctypes.c_int), ("diis", ctypes.c_int),
+                ("diag", ctypes.c_int), ("renormalize", ctypes.c_int), ("schwarz", ctypes.c_double),
+                ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("e_elec", ctypes.c_double), ("e_nuc", ctypes.c_double),
+                ("last_dE", ctypes.c_double), ("last_dD", ctypes.c_double), ("iterations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("fock_builds_fp64", ctypes.c_int), ("fock_builds_fp32", ctypes.c_int),
+                ("nbf", ctypes.c_int), ("nshell", ctypes.c_int), ("t_total", ctypes.c_double),
+                ("t_setup", ctypes.c_double), ("t_fock", ctypes.c_double), ("t_diag", ctypes.c_double),
+                ("quartets", ctypes.c_double), ("min_overlap_eig", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_P = ctypes.POINTER
+_d, _i = ctypes.c_double, ctypes.c_int
+_pd, _pi32 = _P(ctypes.c_double), _P(ctypes.c_int32)
+_SIGS = {
+    "run_scf_c": (None, [_i, _i, _i, _i, _i, ctypes.c_char_p, _pd, _d, _i, _pd, ctypes.c_char_p, _pi32]),
+    "pf_default_options": (None, [_P(Options)]),
+    "pf_last_error": (ctypes.c_char_p, []),
+    "pf_device_count": (_i, []),
+    "pf_basis_info": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _P(_i), _P(_i), _P(_i), _P(_i),
+                           _P(_i), _pd, _pd]),
+    "pf_one_electron": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd, _pd, _pd]),
+    "pf_eri_packed": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd]),
+    "pf_jk": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pd, _pd, _pd, _i, _i, _d]),
+    "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
+    "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i)]),
+    "pf_boys": (_i, [_i, _i, _pd, _pd]),
+    "pf_eigh": (_i, [_i, _pd, _pd, _pd]),
+    "pf_projector": (_i, [_i, _pd, _d, _pd]),
+    "pf_gemm": (_i, [_i, _i, _i, _i, _i, _d, _pd, _pd, _d, _pd]),
+}
+
+
+def load_library(path=None):
+    """Load libpyfacc.so (raises ``PyfaccError`` if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PyfaccError(f"{p} is missing: build it with `python pyfacc-scf_amd/build.py` "
+                          "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load_library().pf_last_error()
+        raise PyfaccError((msg or b"error").decode() + f" (status {rc})")
+
+
+def device_count():
+    return load_library().pf_device_count()
+
+
+def default_options(**kw):
+    o = Options()
+    load_library().pf_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class _Mol:
+    """Marshal (species, coords_bohr) into the C-ABI's argument layout."""
+
+    def __init__(self, species, coords_bohr, z=None):
+        from .helper_functions import symbols_to_z
+        self.species = [str(s) for s in species]
+        self.n = len(self.species)
+        xyz = np.asarray(coords_bohr, dtype=np.float64)
+        if xyz.shape != (self.n, 3):
+            raise ValueError(f"coords must have shape (natoms, 3) = ({self.n}, 3), got {xyz.shape}")
+        self.xyz = np.ascontiguousarray(xyz)
+        self.z = np.ascontiguousarray(np.array(z if z is not None else symbols_to_z(self.species), dtype=np.int32))
+        self.sp = b"".join(s.ljust(2).encode() for s in self.species)
+
+    def args(self):
+        return (self.n, self.sp, self.z.ctypes.data_as(_pi32), self.xyz.ctypes.data_as(_pd))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+def basis_info(species, coords_bohr, basis="sto-3g", renormalize=True):
+    m = _Mol(species, coords_bohr)
+    lib = load_library()
+    nbf, nsh = _i(0), _i(0)
+    _check(lib.pf_basis_info(*m.args(), basis.encode(), int(renormalize), ctypes.byref(nbf), ctypes.byref(nsh),
+                             None, None, None, None, None))
+    L = np.zeros(nsh.value, np.int32)
+    atom = np.zeros(nsh.value, np.int32)
+    npr = np.zeros(nsh.value, np.int32)
+    ex = np.zeros(nsh.value * 8)
+    co = np.zeros(nsh.value * 8)
+    ip = lambda a: a.ctypes.data_as(_P(_i))
+    _check(lib.pf_basis_info(*m.args(), basis.encode(), int(renormalize), ctypes.byref(nbf), ctypes.byref(nsh),
+                             ip(L), ip(atom), ip(npr), _ptr(ex), _ptr(co)))
+    return dict(nbf=nbf.value, nshell=nsh.value, l=L, atom=atom, nprim=npr, exps=ex.reshape(-1, 8),
+                coefs=co.reshape(-1, 8))
+
+
+def one_electron(species, coords_bohr, basis="sto-3g", renormalize=True):
+    n = basis_info(species, coords_bohr, basis, renormalize)["nbf"]
+    m = _Mol(species, coords_bohr)
+    S, T, V = (np.zeros((n, n)) for _ in range(3))
+    _check(load_library().pf_one_electron(*m.args(), basis.encode(), int(renormalize), _ptr(S), _ptr(T), _ptr(V)))
+    return S, T, V
+
+
+def eri_packed(species, coords_bohr, basis="sto-3g", renormalize=True):
+    n = basis_info(species, coords_bohr, basis, renormalize)["nbf"]
+    npair = n * (n + 1) // 2
+    out = np.zeros(npair * (npair + 1) // 2)
+    m = _Mol(species, coords_bohr)
+    _check(load_library().pf_eri_packed(*m.args(), basis.encode(), int(renormalize), _ptr(out)))
+    return out
+
+
+def jk(species, coords_bohr, D, basis="sto-3g", renormalize=True, stored=False, fp32=False, schwarz=0.0):
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    nd = 1 if D.ndim == 2 else D.shape[0]
+    n = D.shape[-1]
+    J = np.zeros((n, n))
+    K = np.zeros_like(D)
+    m = _Mol(species, coords_bohr)
+    _check(load_library().pf_jk(*m.args(), basis.encode(), int(renormalize), nd, _ptr(D), _ptr(J), _ptr(K),
+                                PF_ERI_STORED if stored else PF_ERI_DIRECT,
+                                PF_PREC_MIXED if fp32 else PF_PREC_FP64, float(schwarz)))
+    return J, K
+
+
+def scf(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, want_density=False, want_eps=False,
+        **kw):
+    """Run an SCF; returns a dict of pf_result fields (+ D, eps on request)."""
+    opt = options if options is not None else default_options(**kw)
+    m = _Mol(species, coords_bohr)
+    n = basis_info(species, coords_bohr, basis, bool(opt.renormalize))["nbf"]
+    nd = 1 if nalpha == nbeta else 2
+    D = np.zeros((nd, n, n)) if want_density else None
+    eps = np.zeros((nd, n)) if want_eps else None
+    r = Result()
+    _check(load_library().pf_scf(*m.args(), basis.encode(), int(nalpha), int(nbeta), ctypes.byref(opt),
+                                 ctypes.byref(r), _ptr(D), _ptr(eps)))
+    out = r.as_dict()
+    if want_density:
+        out["D"] = D if nd == 2 else D[0]
+    if want_eps:
+        out["eps"] = eps
+    return out
+
+
+def scf_profile():
+    """(quartets, ms, launches), each shaped (2, 6): [FP64, FP32] x class."""
+    q = (ctypes.c_ulonglong * 12)()
+    ms = np.zeros(12)
+    n = np.zeros(12, dtype=np.int32)
+    load_library().pf_scf_profile(q, _ptr(ms), n.ctypes.data_as(_P(_i)))
+    return np.array(list(q), dtype=np.int64).reshape(2, 6), ms.reshape(2, 6), n.reshape(2, 6)
+
+
+def boys(nmax, x):
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    F = np.zeros((len(x), nmax + 1))
+    _check(load_library().pf_boys(nmax, len(x), _ptr(x), _ptr(F)))
+    return F
+
+
+def eigh(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    _check(load_library().pf_eigh(n, _ptr(A), _ptr(w), _ptr(V)))
+    return w, V
+
+
+def projector(F, nocc):
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    P = np.zeros_like(F)
+    _check(load_library().pf_projector(F.shape[0], _ptr(F), float(nocc), _ptr(P)))
+    return P
+
+
+def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    m = A.shape[1] if ta else A.shape[0]
+    k = A.shape[0] if ta else A.shape[1]
+    n = B.shape[0] if tb else B.shape[1]
+    C = np.zeros((m, n)) if C is None else np.ascontiguousarray(C, dtype=np.float64).copy()
+    _check(load_library().pf_gemm(m, n, k, int(ta), int(tb), alpha, _ptr(A), _ptr(B), beta, _ptr(C)))
+    return C

@@ -1,0 +1,99 @@
+"""GPU SCF energies against the oracle and literature values.
+
+Energy parity bar: 1e-9 Eh against the FP64 oracle on the same geometry,
+basis and iteration scheme.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, load_xyz
+
+pytestmark = pytest.mark.gpu
+
+CRAWFORD_XYZ = np.array([[0.0, -0.143225816552, 0.0], [1.638036840407, 1.136548822547, 0.0],
+                         [-1.638036840407, 1.136548822547, 0.0]])
+
+
+def test_crawford_known_answer(gpu_lib):
+    """H2O STO-3G, Crawford programming project #3: E = -74.942079928192 Eh
+    (computed with the 8-digit Hehre-Stewart-Pople STO-3G values)."""
+    import pyfacc
+    basis = os.path.join(GOLDEN, "sto-3g-hsp1969.txt")
+    r = pyfacc.scf(["O", "H", "H"], CRAWFORD_XYZ, 5, 5, basis=basis, eps=1e-12, max_iter=200)
+    assert r["converged"]
+    assert abs(r["e_nuc"] - 8.002367061810450) < 1e-12
+    assert abs(r["energy"] - (-74.942079928192)) < 1e-9
+
+
+@pytest.mark.parametrize("diag", [0, 1])
+def test_h2o_rhf_plain_roothaan_vs_oracle(gpu_lib, diag):
+    import oracle
+    import pyfacc
+    sp, xyz = load_xyz("h2o_c1.xyz")
+    r = pyfacc.scf(sp, xyz, 5, 5, eps=1e-10, max_iter=200, diis=0, diag=diag)
+    ref = oracle.Molecule(sp, xyz).scf(5, 5, eps=1e-10, max_iter=200)
+    assert r["converged"] and ref["converged"]
+    assert r["iterations"] == ref["iterations"]
+    assert abs(r["energy"] - ref["energy"]) < 1e-9
+
+
+def test_oh_uhf_vs_oracle(gpu_lib):
+    import oracle
+    import pyfacc
+    sp, xyz = load_xyz("oh.xyz")
+    r = pyfacc.scf(sp, xyz, 5, 4, eps=1e-10, max_iter=300, diis=0)
+    ref = oracle.Molecule(sp, xyz).scf(5, 4, eps=1e-10, max_iter=300)
+    assert r["converged"] and ref["converged"]
+    assert abs(r["energy"] - ref["energy"]) < 1e-9
+
+
+def test_max_iter_unconverged_matches_oracle(gpu_lib):
+    """Reference behaviour: stop silently after max_iter (single_file.py uses 10)."""
+    import oracle
+    import pyfacc
+    sp, xyz = load_xyz("h2o_c1.xyz")
+    r = pyfacc.scf(sp, xyz, 5, 5, eps=1e-10, max_iter=3, diis=0)
+    ref = oracle.Molecule(sp, xyz).scf(5, 5, eps=1e-10, max_iter=3)
+    assert not r["converged"] and r["iterations"] == 3
+    assert abs(r["energy"] - ref["energy"]) < 1e-9
+
+
+def test_dropin_run_scf(gpu_lib, tmp_path, monkeypatch):
+    """The reference driver's call path: run_scf(lib, 'h2o.xyz', ...) -> run_scf_c."""
+    import oracle
+    import pyfacc
+    from pyfacc.helper_functions import read_xyz, run_scf
+    monkeypatch.chdir(tmp_path)  # no ./data here: package data is used
+    lib = ctypes.CDLL(pyfacc.LIB_PATH)
+    assert hasattr(lib, "run_scf_interface_py")
+    e = run_scf(lib, "h2o.xyz", 1e-10, 100, ctypes.c_double(), "sto-3g")
+    n, coords, species = read_xyz("h2o.xyz")
+    ref = oracle.Molecule(list(species), coords.T).scf(5, 5, eps=1e-10, max_iter=100)
+    assert abs(e - ref["energy"]) < 1e-9
+
+
+def test_dropin_error_is_nan_and_message(gpu_lib):
+    import pyfacc
+    lib = ctypes.CDLL(pyfacc.LIB_PATH)
+    lib.pf_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_double(0.0)
+    z = (ctypes.c_int32 * 1)(1)
+    xyz = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
+    lib.run_scf_c(1, 1, 0, 1, 0, b"H ", xyz, ctypes.c_double(1e-8), 10, ctypes.byref(out), b"no-such-basis", z)
+    assert out.value != out.value
+    assert b"no-such-basis" in lib.pf_last_error()
+
+
+@pytest.mark.parametrize("precision,incremental", [(0, 1), (1, 0)])
+def test_water_dimer_modes_agree(gpu_lib, precision, incremental):
+    """Incremental FP64 and mixed precision converge to the full-FP64 energy."""
+    import pyfacc
+    sp, xyz = load_xyz("h2o_16.xyz")
+    sp, xyz = sp[:6], xyz[:6]
+    base = pyfacc.scf(sp, xyz, 10, 10, eps=1e-9)
+    r = pyfacc.scf(sp, xyz, 10, 10, eps=1e-9, precision=precision, incremental=incremental)
+    assert base["converged"] and r["converged"]
+    assert abs(r["energy"] - base["energy"]) < 1e-8
