@@ -115,13 +115,13 @@ def main():
     t0 = time.perf_counter()
     mixed = [pyfacc.scf(species, xyz, nocc, nocc, options=optmx) for _ in range(args.steps)]
     t_mixed = time.perf_counter() - t0
-    q_mx, ms_mx, n_mx = pyfacc.scf_profile()
+    q_mx, ms_mx, n_mx, orient = pyfacc.scf_profile()
 
     # ---- comparison: plain FP64 and FP64-incremental, timed the same way
     t0 = time.perf_counter()
     fp64 = [pyfacc.scf(species, xyz, nocc, nocc, options=opt64) for _ in range(args.steps)]
     t_fp64 = time.perf_counter() - t0
-    q_64, ms_64, n_64 = pyfacc.scf_profile()
+    q_64, ms_64, n_64, orient64 = pyfacc.scf_profile()
     t0 = time.perf_counter()
     inc = pyfacc.scf(species, xyz, nocc, nocc, options=optinc)
     t_inc = time.perf_counter() - t0
@@ -135,13 +135,20 @@ def main():
     launches = int(n_mx[prec, cls])
     avg_ms = flat_ms[k] / max(launches, 1)
     q_per_launch = q_mx[prec, cls] / max(launches, 1)
-    fl = flops.class_flops(cls, q_per_launch, kprim_pair=9, ndens=1, taylor_terms=7 if prec == 0 else 5)
+    fl = flops.class_flops(cls, q_per_launch, kprim_pair=9, ndens=1, taylor_terms=7 if prec == 0 else 5,
+                           orient=orient[cls])
     achieved = fl / (avg_ms * 1e-3) / 1e12
     # the same kernel class in the plain FP64 SCF (full builds)
     k64 = int(np.argmax(ms_64[0]))
     avg64 = ms_64[0, k64] / max(n_64[0, k64], 1)
-    fl64 = flops.class_flops(k64, q_64[0, k64] / max(n_64[0, k64], 1))
-    fock64_flops = flops.fock_flops(q_64[0] / np.maximum(n_64[0], 1))
+    fl64 = flops.class_flops(k64, q_64[0, k64] / max(n_64[0, k64], 1), orient=orient64[k64])
+    fock64_flops = flops.fock_flops(q_64[0] / np.maximum(n_64[0], 1), orients=orient64)
+    per_class = {flops.CLASS_NAMES[c]: {
+        "orient": "".join("sp"[int(x)] for x in orient64[c]),
+        "quartets_per_build": int(q_64[0, c] / max(n_64[0, c], 1)),
+        "ms_per_build": float(ms_64[0, c] / max(n_64[0, c], 1)),
+        "tflops": float(flops.class_flops(c, q_64[0, c] / max(n_64[0, c], 1), orient=orient64[c]) /
+                        max(ms_64[0, c] / max(n_64[0, c], 1) * 1e-3, 1e-12) / 1e12)} for c in range(6)}
 
     out = {
         "metric": "SCF time-to-converge & Fock builds/s, mixed vs FP64 GPU at |dE|<=1e-6 Eh",
@@ -174,6 +181,7 @@ def main():
         "fock_build_ms_fp64": 1e3 * fp64[-1]["t_fock"] / fp64[-1]["fock_builds_fp64"],
         "fock_build_tflops_fp64": fock64_flops / (fp64[-1]["t_fock"] / fp64[-1]["fock_builds_fp64"]) / 1e12,
         "converged": bool(mixed[-1]["converged"] and fp64[-1]["converged"]),
+        "fp64_build_per_class": per_class,
         "roofline": {
             "bound": "mfma",
             "kernel": f"jk_kernel {flops.CLASS_NAMES[cls]} {'fp64' if prec == 0 else 'fp32'}",

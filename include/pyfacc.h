@@ -130,8 +130,10 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
  * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,
  * 1 FP32; classes (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)):
  * quartets that passed screening, kernel milliseconds (HIP events on the
- * engine's stream) and kernel launches.  Any output may be NULL. */
-int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches);
+ * engine's stream) and kernel launches; class_l[24] gives per class the
+ * angular momenta (la lb | lc ld) in the orientation launched.  Any output
+ * may be NULL. */
+int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l);
 
 /* Device kernels exposed for testing on host buffers. */
 int pf_boys(int nmax, int npts, const double *x, double *F);               /* F[npts*(nmax+1)], nmax <= 6 */

@@ -22,7 +22,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
-SOURCES = ["eri.hip", "onee.hip", "linalg.hip", "engine.hip", "testkern.hip", "basis.cpp", "abi.cpp"]
+ERI_CLASSES = ["ssss", "psss", "ssps", "psps", "ppss", "sspp", "ppps", "pspp", "pppp"]
+SOURCES = ["eri_%s.hip" % c for c in ERI_CLASSES] + ["eri.hip", "onee.hip", "linalg.hip", "engine.hip", "testkern.hip", "basis.cpp", "abi.cpp"]
 
 
 def _compile(src):
@@ -38,7 +39,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(int(os.environ.get('MAX_JOBS', '8')), len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-ldl"]

@@ -17,6 +17,7 @@ thread_local std::string g_last_error;
 thread_local std::array<unsigned long long, 12> g_prof_q{};
 thread_local std::array<double, 12> g_prof_ms{};
 thread_local std::array<int, 12> g_prof_n{};
+thread_local std::array<int, 24> g_prof_l{};
 
 struct Input {
     Molecule mol;
@@ -165,16 +166,19 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
         g_prof_q = eng.prof_quartets;
         g_prof_ms = eng.prof_ms;
         g_prof_n = eng.prof_launches;
+        g_prof_l = eng.prof_class_l;
         if (res) *res = r;
     });
 }
 
-int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches) {
+int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l) {
     for (int q = 0; q < 12; ++q) {
         if (quartets) quartets[q] = g_prof_q[q];
         if (ms) ms[q] = g_prof_ms[q];
         if (launches) launches[q] = g_prof_n[q];
     }
+    for (int q = 0; q < 24; ++q)
+        if (class_l) class_l[q] = g_prof_l[q];
     return PF_OK;
 }
 

@@ -16,20 +16,18 @@ __host__ __device__ constexpr int coff(int L) { return L * (L + 1) * (L + 2) / 6
 __host__ __device__ constexpr int cidx(int lx, int ly, int lz) {
     return coff(lx + ly + lz) + (ly + lz) * (ly + lz + 1) / 2 + lz;
 }
+// Loop-free so that, inside fully unrolled recurrences, every index folds to a
+// constant before scalar replacement of the register arrays (L <= 5).
 __host__ __device__ constexpr int cL(int g) {
-    int L = 0;
-    while (coff(L + 1) <= g) ++L;
-    return L;
+    return (g >= 1) + (g >= 4) + (g >= 10) + (g >= 20) + (g >= 35) + (g >= 56);
 }
-__host__ __device__ constexpr int cx(int g) {
-    int L = cL(g), pos = g - coff(L), r = 0;
-    while ((r + 1) * (r + 2) / 2 <= pos) ++r;
-    return L - r;
+// r = number of components of a triangle row index pos: largest r with r(r+1)/2 <= pos
+__host__ __device__ constexpr int ctri(int pos) {
+    return (pos >= 1) + (pos >= 3) + (pos >= 6) + (pos >= 10) + (pos >= 15);
 }
+__host__ __device__ constexpr int cx(int g) { return cL(g) - ctri(g - coff(cL(g))); }
 __host__ __device__ constexpr int cz(int g) {
-    int L = cL(g), pos = g - coff(L), r = 0;
-    while ((r + 1) * (r + 2) / 2 <= pos) ++r;
-    return pos - r * (r + 1) / 2;
+    return (g - coff(cL(g))) - ctri(g - coff(cL(g))) * (ctri(g - coff(cL(g))) + 1) / 2;
 }
 __host__ __device__ constexpr int cy(int g) { return cL(g) - cx(g) - cz(g); }
 __host__ __device__ constexpr int ccomp(int g, int d) { return d == 0 ? cx(g) : (d == 1 ? cy(g) : cz(g)); }
@@ -48,5 +46,6 @@ static_assert(cidx(2, 0, 0) == 4 && cidx(1, 1, 0) == 5 && cidx(1, 0, 1) == 6 && 
                   cidx(0, 1, 1) == 8 && cidx(0, 0, 2) == 9, "d order");
 static_assert(cx(6) == 1 && cy(6) == 0 && cz(6) == 1 && cx(8) == 0 && cy(8) == 1 && cz(8) == 1, "inverse");
 static_assert(cminus(cidx(1, 1, 0), 0) == cidx(0, 1, 0), "minus");
+static_assert(cx(19) == 0 && cy(19) == 0 && cz(19) == 3 && cx(10) == 3 && cL(34) == 4 && cz(34) == 4, "f, g");
 
 }  // namespace pf

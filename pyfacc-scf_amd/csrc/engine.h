@@ -29,6 +29,7 @@ struct PairSet {
 
 struct QuartetClass {
     int bra, ket;  // indices into Engine::pairs_
+    int slot;      // unordered class: (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)
 };
 
 struct Timing {
@@ -62,6 +63,7 @@ class Engine {
     std::array<unsigned long long, 12> prof_quartets{};
     std::array<double, 12> prof_ms{};
     std::array<int, 12> prof_launches{};
+    std::array<int, 24> prof_class_l{};  // per slot: la lb lc ld as launched (bra | ket)
     bool profile_classes = false;
 
    private:

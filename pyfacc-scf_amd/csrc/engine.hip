@@ -185,9 +185,19 @@ void Engine::setup_pairs() {
         for (auto &v : fq) v = std::nextafter(v, 1e30f);
         ps.f_Q.upload(fq.data(), fq.size(), st_);
     }
+    // One launch per unordered class; for mixed classes the bra is the pair
+    // class with more pairs, so the grid (one wavefront per bra pair) is wide
+    // and the longer ket loop runs over the shorter list.
     classes_.clear();
+    int slot = 0;
     for (int x = 0; x < 3; ++x)
-        for (int y = 0; y <= x; ++y) classes_.push_back({x, y});
+        for (int y = 0; y <= x; ++y, ++slot) {
+            QuartetClass qc{x, y, slot};
+            if (x != y && pairs_[y].n > pairs_[x].n) std::swap(qc.bra, qc.ket);
+            classes_.push_back(qc);
+            const int l4[4] = {pairs_[qc.bra].la, pairs_[qc.bra].lb, pairs_[qc.ket].la, pairs_[qc.ket].lb};
+            for (int k = 0; k < 4; ++k) prof_class_l[slot * 4 + k] = l4[k];
+        }
     const size_t n2 = (size_t)nbf() * nbf();
     J64_.alloc(n2); K64a_.alloc(n2); K64b_.alloc(n2); Dj64_.alloc(n2);
     Dsh64_.alloc((size_t)nshell() * nshell()); dmax64_.alloc(1);
@@ -287,6 +297,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
     hipEvent_t ev[7];
     if (profile_classes)
         for (auto &e : ev) PF_HIP(hipEventCreate(&e));
+    static_assert(sizeof(ev) / sizeof(ev[0]) == 7, "one event per class + 1");
     for (size_t q = 0; q < classes_.size(); ++q) {
         const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
         if constexpr (std::is_same<R, double>::value) { a.bra = B.view64(); a.ket = K.view64(); }
@@ -297,7 +308,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
                 a.store = store_[q].p; a.store_off = store_off_[q].p; a.store_nket = store_nket_[q].p;
             }
         }
-        a.count = count_.p + q;
+        a.count = count_.p + classes_[q].slot;
         if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
@@ -317,7 +328,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         for (int q = 0; q < 6; ++q) {
             float ms = 0.f;
             PF_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
-            prof_ms[pbase + q] += ms;
+            prof_ms[pbase + classes_[q].slot] += ms;
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
@@ -479,8 +490,11 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         const double tf = now();
         const double *Dcur0 = D.p, *Dcur1 = uhf ? D.p + n2 : nullptr;
         if (fp32_phase && last_dD < opt.mixed_switch) {
-            // leave FP32: one full FP64 build, FP64 increments afterwards
+            // leave FP32: one full FP64 build, FP64 increments afterwards; the
+            // DIIS subspace holds FP32-era Fock matrices and is restarted
             fp32_phase = false;
+            nstore = 0;
+            head = 0;
             res.quartets += build_g<double>(Dcur0, Dcur1, G.p, uhf ? G.p + n2 : nullptr, false, gmode);
             ++res.fock_builds_fp64;
             PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));

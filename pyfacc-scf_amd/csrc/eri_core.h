@@ -49,21 +49,18 @@ struct Eri {
     // Compact storage of [g|h]^m: only m <= L - L(g) - L(h) is ever needed.
     static constexpr int nm(int g, int h) { return L - cL(g) - cL(h) + 1; }
     // Offset of [g|h]^0 in the compact array: entries are laid out g-major,
-    // h-minor, each (g, h) holding nm(g, h) orders.  Closed form with loops of
-    // at most LAB+1 / LCD+1 trips so it folds to a constant after unrolling.
-    static constexpr int row(int Lg) {  // entries of one g of angular momentum Lg
-        int s = 0;
-        for (int Lh = 0; Lh <= LCD; ++Lh) s += nc(Lh) * (L - Lg - Lh + 1);
-        return s;
-    }
+    // h-minor, each (g, h) holding nm(g, h) orders.  Closed form (no loops),
+    // so it folds to a constant inside the unrolled recurrences:
+    //   row(l)    = sum_{Lh<=LCD} nc(Lh) (L - l - Lh + 1) = (L - l + 1) coff(LCD+1) - S1(LCD+1)
+    //   S1(n)     = sum_{l<n} l nc(l) = (n-1) n (n+1) (n+2) / 8
+    //   base(g,h) = sum_{l<Lg} nc(l) row(l) + (g - coff(Lg)) row(Lg)
+    //             + (L - Lg + 1) coff(Lh) - S1(Lh) + (h - coff(Lh)) (L - Lg - Lh + 1)
+    static constexpr int S1(int n) { return (n - 1) * n * (n + 1) * (n + 2) / 8; }
+    static constexpr int row(int l) { return (L - l + 1) * coff(LCD + 1) - S1(LCD + 1); }
     static constexpr int base(int g, int h) {
         const int Lg = cL(g), Lh = cL(h);
-        int s = 0;
-        for (int l = 0; l < Lg; ++l) s += nc(l) * row(l);
-        s += (g - coff(Lg)) * row(Lg);
-        for (int l = 0; l < Lh; ++l) s += nc(l) * (L - Lg - l + 1);
-        s += (h - coff(Lh)) * (L - Lg - Lh + 1);
-        return s;
+        return coff(Lg) * ((L + 1) * coff(LCD + 1) - S1(LCD + 1)) - coff(LCD + 1) * S1(Lg) +
+               (g - coff(Lg)) * row(Lg) + (L - Lg + 1) * coff(Lh) - S1(Lh) + (h - coff(Lh)) * (L - Lg - Lh + 1);
     }
     static constexpr int NTH = base(GE - 1, GF - 1) + nm(GE - 1, GF - 1);
 
@@ -175,4 +172,6 @@ static_assert(Eri<1, 1, 1, 1, double>::base(0, 0) == 0 && Eri<1, 1, 1, 1, double
                   Eri<1, 1, 1, 1, double>::base(1, 0) == 5 + 3 * 4 + 6 * 3,
               "compact VRR layout");
 static_assert(Eri<1, 1, 1, 1, double>::NTH == 200, "compact VRR size");
+static_assert(Eri<1, 0, 1, 1, double>::NTH == 70 && Eri<0, 0, 0, 0, double>::NTH == 1, "compact VRR size");
+static_assert(Eri<1, 1, 1, 1, double>::base(9, 9) == 199 && Eri<1, 0, 1, 1, double>::base(3, 9) == 69, "last entry");
 }  // namespace pf

@@ -51,7 +51,7 @@ _SIGS = {
     "pf_eri_packed": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd]),
     "pf_jk": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pd, _pd, _pd, _i, _i, _d]),
     "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
-    "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i)]),
+    "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i)]),
     "pf_boys": (_i, [_i, _i, _pd, _pd]),
     "pf_eigh": (_i, [_i, _pd, _pd, _pd]),
     "pf_projector": (_i, [_i, _pd, _d, _pd]),
@@ -189,12 +189,15 @@ def scf(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, want_
 
 
 def scf_profile():
-    """(quartets, ms, launches), each shaped (2, 6): [FP64, FP32] x class."""
+    """(quartets, ms, launches) each shaped (2, 6) = [FP64, FP32] x class, and
+    the (6, 4) angular momenta (la lb lc ld) of each class as launched."""
     q = (ctypes.c_ulonglong * 12)()
     ms = np.zeros(12)
     n = np.zeros(12, dtype=np.int32)
-    load_library().pf_scf_profile(q, _ptr(ms), n.ctypes.data_as(_P(_i)))
-    return np.array(list(q), dtype=np.int64).reshape(2, 6), ms.reshape(2, 6), n.reshape(2, 6)
+    cl = np.zeros(24, dtype=np.int32)
+    load_library().pf_scf_profile(q, _ptr(ms), n.ctypes.data_as(_P(_i)), cl.ctypes.data_as(_P(_i)))
+    return (np.array(list(q), dtype=np.int64).reshape(2, 6), ms.reshape(2, 6), n.reshape(2, 6),
+            cl.reshape(6, 4))
 
 
 def boys(nmax, x):
