@@ -61,11 +61,10 @@ def cpu_baseline(species, xyz, n_builds, target_s):
     # loop overhead of visiting every pair of pairs (no integrals), then a
     # calibration sample, then a sample sized to ~target_s seconds
     t_loop, _ = timed(1 << 62)
-    stride = 4000
+    t_cal, done = timed(512)
+    full_est = max(t_cal - t_loop, 1e-3) * 512
+    stride = max(2, int(np.ceil(full_est / target_s)))
     t_cal, done = timed(stride)
-    if t_cal - t_loop < target_s / 4:
-        stride = max(1, int(stride * max(t_cal - t_loop, 1e-2) / target_s))
-        t_cal, done = timed(stride)
     t_build = max(t_cal - t_loop, 1e-9) * stride + t_loop
     return {
         "value": 1.0 / (t_build * n_builds),

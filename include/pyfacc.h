@@ -65,6 +65,8 @@ typedef struct pf_options {
     double schwarz;       /* screening threshold on Q_ab Q_cd max|D| */
     double mixed_switch;  /* mixed precision: ||dD||_F below which the FP64 finish starts */
     int verbose;          /* 1: one line per iteration on stderr */
+    double schwarz_inc;   /* screening threshold of FP64 incremental builds (on Q Q max|dD|) */
+    int rebuild_every;    /* incremental FP64 builds: full rebuild after this many increments (0: never) */
 } pf_options;
 
 typedef struct pf_result {
@@ -88,7 +90,8 @@ typedef struct pf_result {
 } pf_result;
 
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
- * renormalised, Schwarz 1e-10, mixed_switch 1e-5. */
+ * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
+ * rebuild_every 0. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */
@@ -126,6 +129,14 @@ int pf_jk(int natoms, const char *species, const int32_t *z, const double *coord
 int pf_scf(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
            int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *D_out, double *eps_out);
 
+/* SCF (as pf_scf) followed by the analytic nuclear gradient dE/dR at the
+ * converged density: grad[natoms*3] in Eh/bohr (one-electron, two-electron
+ * and nuclear-repulsion terms; W = 1/2 D F D, RHF, or sum_s D_s F_s D_s, UHF).
+ * The reference has no gradients; this is the north star's differentiation
+ * extension, bound to torch.autograd by pyfacc.grad.  D_out may be NULL. */
+int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                    int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *grad, double *D_out);
+
 /* Fock-build profile of the last pf_scf call on this thread, summed over its
  * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,
  * 1 FP32; classes (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)):
@@ -141,6 +152,9 @@ int pf_eigh(int n, const double *A, double *w, double *V);                  /* J
 int pf_projector(int n, const double *F, double nocc, double *P);           /* SP2 spectral projector */
 int pf_gemm(int m, int n, int k, int ta, int tb, double alpha, const double *A, const double *B, double beta,
             double *C);                                                     /* row-major, FP64 MFMA */
+/* Plain device-to-device copy of `bytes` (kernel ms in *ms): the reference
+ * kernel for calibrating HBM byte counters. */
+int pf_device_copy(unsigned long long bytes, double *ms);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 
 namespace pf {
 void boys_device(int nmax, int npts, const double *x, double *F, hipStream_t st);
+double device_copy(size_t bytes, hipStream_t st);
 
 namespace {
 thread_local std::string g_last_error;
@@ -89,6 +90,8 @@ void pf_default_options(pf_options *o) {
     o->schwarz = 1e-10;
     o->mixed_switch = 1e-5;
     o->verbose = 0;
+    o->schwarz_inc = 1e-12;
+    o->rebuild_every = 0;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }
@@ -171,6 +174,24 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
     });
 }
 
+int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+                    int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *grad, double *D_out) {
+    return guarded([&] {
+        pf_options o;
+        if (opt) o = *opt;
+        else pf_default_options(&o);
+        if (o.max_iter < 1) throw Error("max_iter must be >= 1");
+        if (!grad) throw Error("null gradient output");
+        Input in = make_input(natoms, species, z, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,
+                   o.schwarz > 0 ? o.schwarz : 1e-14);
+        pf_result r;
+        eng.scf(nalpha, nbeta, o, r, D_out, nullptr);
+        eng.gradient(grad, 1e-13);
+        if (res) *res = r;
+    });
+}
+
 int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l) {
     for (int q = 0; q < 12; ++q) {
         if (quartets) quartets[q] = g_prof_q[q];
@@ -243,6 +264,17 @@ int pf_gemm(int m, int n, int k, int ta, int tb, double alpha, const double *A, 
             dC.download(C, (size_t)m * n, st);
             PF_HIP(hipStreamSynchronize(st));
         }
+        PF_HIP(hipStreamDestroy(st));
+    });
+}
+
+int pf_device_copy(unsigned long long bytes, double *ms) {
+    return guarded([&] {
+        if (pf_device_count() == 0) throw Error("no GPU device visible");
+        hipStream_t st;
+        PF_HIP(hipStreamCreate(&st));
+        const double t = device_copy((size_t)bytes, st);
+        if (ms) *ms = t;
         PF_HIP(hipStreamDestroy(st));
     });
 }

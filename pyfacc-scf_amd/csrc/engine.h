@@ -56,6 +56,9 @@ class Engine {
 
     // Full SCF.  D_out (ndens x nbf^2) and eps_out (ndens x nbf) may be null.
     void scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out);
+    // Nuclear gradient (natoms x 3, Eh/bohr) at the density of the last scf()
+    // call; two-electron quartets screened at tau on Q Q max|Gamma|.
+    void gradient(double *grad_out, double tau);
 
     // Fock-build statistics accumulated over an SCF, index precision*6 + class
     // (precision 0 = FP64, 1 = FP32): quartets, kernel milliseconds (HIP events
@@ -70,10 +73,12 @@ class Engine {
     void setup_one_electron();
     void setup_pairs();
     void setup_store();
+    void setup_grad_pairs();
     // G (FP64) += or = G[D] in precision R.  D0/D1 device FP64 densities (D1 null for RHF).
+    // tau > 0 overrides the Schwarz threshold (used for incremental builds).
     template <typename R>
     unsigned long long build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
-                               int mode);
+                               int mode, double tau = 0.0);
     void density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
                            double *eps_dev);
 
@@ -94,7 +99,7 @@ class Engine {
     bool one_e_ready_ = false;
     // J/K accumulators
     DevBuf<double> J64_, K64a_, K64b_, Dsh64_, dmax64_;
-    DevBuf<float> J32_, K32a_, K32b_, Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;
+    DevBuf<float> Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;
     DevBuf<double> Dj64_;
     DevBuf<unsigned long long> count_;
     // stored integrals
@@ -104,6 +109,12 @@ class Engine {
     std::array<DevBuf<int>, 6> store_nket_;
     // work
     DevBuf<double> w1_, w2_, w3_, state_;
+    // converged densities of the last SCF (for the gradient)
+    DevBuf<double> Dfin_;
+    bool have_fin_ = false, uhf_fin_ = false;
+    // gradient pair data: 2 alpha weighted, reversed 2 beta weighted, reversed
+    bool grad_ready_ = false;
+    std::array<DevBuf<double>, 3> gw_, grw_, gr_;
 };
 
 // Thread-local error text for the C-ABI.

@@ -14,6 +14,7 @@
 
 #include "boys.h"
 #include "engine.h"
+#include "grad_kernel.h"
 #include "linalg.h"
 
 namespace pf {
@@ -250,16 +251,17 @@ void Engine::setup_store() {
 
 template <typename R>
 unsigned long long Engine::build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
-                                   int mode) {
+                                   int mode, double tau) {
     setup_pairs();
     const int n = nbf(), nsh = nshell();
     const size_t n2 = (size_t)n * n;
     const bool uhf = D1 != nullptr;
     JKArgs<R> a;
-    a.nbf = n; a.nsh = nsh; a.tau = R(schwarz_);
-    R *J, *Ka, *Kb;
+    a.nbf = n; a.nsh = nsh; a.tau = R(tau > 0 ? tau : schwarz_);
+    double *J = J64_.p, *Ka = K64a_.p, *Kb = K64b_.p;
+    dev_fill(n2, 0.0, J, st_); dev_fill(n2, 0.0, Ka, st_);
+    if (uhf) dev_fill(n2, 0.0, Kb, st_);
     if constexpr (std::is_same<R, double>::value) {
-        J = J64_.p; Ka = K64a_.p; Kb = K64b_.p;
         if (uhf) {
             dev_axpby(n2, 1.0, D0, 1.0, D1, Dj64_.p, st_);
             a.Dj = Dj64_.p; a.Dk0 = D0; a.Dk1 = D1;
@@ -269,14 +271,11 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         a.tab = d_tab_.p;
         dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, D0, D1, Dsh64_.p, dmax64_.p, st_);
         a.Dsh = Dsh64_.p; a.dmax = dmax64_.p;
-        dev_fill(n2, 0.0, J, st_); dev_fill(n2, 0.0, Ka, st_);
-        if (uhf) dev_fill(n2, 0.0, Kb, st_);
     } else {
-        if (!J32_.p) {
-            J32_.alloc(n2); K32a_.alloc(n2); K32b_.alloc(n2); Dj32_.alloc(n2); Dk32a_.alloc(n2); Dk32b_.alloc(n2);
+        if (!Dj32_.p) {
+            Dj32_.alloc(n2); Dk32a_.alloc(n2); Dk32b_.alloc(n2);
             Dsh32_.alloc((size_t)nsh * nsh); dmax32_.alloc(1);
         }
-        J = J32_.p; Ka = K32a_.p; Kb = K32b_.p;
         if (uhf) {
             dev_to_float(n2, 1.0, D0, D1, Dj32_.p, st_);
             dev_to_float(n2, 1.0, D0, nullptr, Dk32a_.p, st_);
@@ -289,8 +288,6 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         a.tab = f_tab_.p;
         dev_shell_dmax<float>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, D0, D1, Dsh32_.p, dmax32_.p, st_);
         a.Dsh = Dsh32_.p; a.dmax = dmax32_.p;
-        dev_fillf(n2, 0.f, J, st_); dev_fillf(n2, 0.f, Ka, st_);
-        if (uhf) dev_fillf(n2, 0.f, Kb, st_);
     }
     a.J = J; a.K0 = Ka; a.K1 = uhf ? Kb : nullptr;
     PF_HIP(hipMemsetAsync(count_.p, 0, 6 * sizeof(unsigned long long), st_));
@@ -313,7 +310,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
     if (profile_classes) PF_HIP(hipEventRecord(ev[6], st_));
-    dev_assemble_g<R>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
+    dev_assemble_g<double>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
     unsigned long long cnt6[6] = {0, 0, 0, 0, 0, 0};
     PF_HIP(hipMemcpyAsync(cnt6, count_.p, sizeof(cnt6), hipMemcpyDeviceToHost, st_));
     PF_HIP(hipStreamSynchronize(st_));
@@ -353,21 +350,9 @@ void Engine::jk_host(int ndens, const double *D, double *J, double *K, int mode,
     else build_g<float>(dD.p, D1, G0.p, G1.p, false, kmode);
     // symmetrised J', K' -> host
     std::vector<double> j(n2), k0(n2), k1(ndens == 2 ? n2 : 0);
-    if (precision == PF_PREC_FP64) {
-        J64_.download(j.data(), n2, st_);
-        K64a_.download(k0.data(), n2, st_);
-        if (ndens == 2) K64b_.download(k1.data(), n2, st_);
-    } else {
-        std::vector<float> f(n2);
-        J32_.download(f.data(), n2, st_); PF_HIP(hipStreamSynchronize(st_));
-        std::copy(f.begin(), f.end(), j.begin());
-        K32a_.download(f.data(), n2, st_); PF_HIP(hipStreamSynchronize(st_));
-        std::copy(f.begin(), f.end(), k0.begin());
-        if (ndens == 2) {
-            K32b_.download(f.data(), n2, st_); PF_HIP(hipStreamSynchronize(st_));
-            std::copy(f.begin(), f.end(), k1.begin());
-        }
-    }
+    J64_.download(j.data(), n2, st_);
+    K64a_.download(k0.data(), n2, st_);
+    if (ndens == 2) K64b_.download(k1.data(), n2, st_);
     PF_HIP(hipStreamSynchronize(st_));
     for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c) {
@@ -483,36 +468,40 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     const bool incremental = opt.incremental || opt.precision == PF_PREC_MIXED;
     const int gmode = opt.eri_mode == PF_ERI_STORED ? MODE_LOAD : MODE_DIRECT;
     double Eold = 0.0, Eel = 0.0, last_dD = 1e300;
-    int it = 0;
+    int it = 0, since_full = 0;
     bool conv = false;
     for (it = 1; it <= opt.max_iter; ++it) {
         // ---- Fock build
         const double tf = now();
         const double *Dcur0 = D.p, *Dcur1 = uhf ? D.p + n2 : nullptr;
-        if (fp32_phase && last_dD < opt.mixed_switch) {
-            // leave FP32: one full FP64 build, FP64 increments afterwards; the
-            // DIIS subspace holds FP32-era Fock matrices and is restarted
-            fp32_phase = false;
-            nstore = 0;
-            head = 0;
-            res.quartets += build_g<double>(Dcur0, Dcur1, G.p, uhf ? G.p + n2 : nullptr, false, gmode);
+        double *G0 = G.p, *G1 = uhf ? G.p + n2 : nullptr;
+        const bool switch_now = fp32_phase && last_dD < opt.mixed_switch;
+        const bool rebuild = !fp32_phase && incremental && opt.rebuild_every > 0 &&
+                             since_full >= opt.rebuild_every;
+        if (switch_now || rebuild || !incremental) {
+            // full FP64 build; in mixed precision this is the switch to FP64, after
+            // which the DIIS subspace (FP32-era Fock matrices) restarts
+            if (switch_now) {
+                fp32_phase = false;
+                nstore = 0;
+                head = 0;
+            }
+            res.quartets += build_g<double>(Dcur0, Dcur1, G0, G1, false, gmode);
             ++res.fock_builds_fp64;
-            PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
-        } else if (incremental) {
+            since_full = 0;
+        } else {
             dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
             const double *d0 = dD.p, *d1 = uhf ? dD.p + n2 : nullptr;
             if (fp32_phase) {
-                res.quartets += build_g<float>(d0, d1, G.p, uhf ? G.p + n2 : nullptr, true, MODE_DIRECT);
+                res.quartets += build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT);
                 ++res.fock_builds_fp32;
             } else {
-                res.quartets += build_g<double>(d0, d1, G.p, uhf ? G.p + n2 : nullptr, true, gmode);
+                res.quartets += build_g<double>(d0, d1, G0, G1, true, gmode, opt.schwarz_inc);
                 ++res.fock_builds_fp64;
+                ++since_full;
             }
-            PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
-        } else {
-            res.quartets += build_g<double>(Dcur0, Dcur1, G.p, uhf ? G.p + n2 : nullptr, false, gmode);
-            ++res.fock_builds_fp64;
         }
+        if (incremental) PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
         for (int s = 0; s < nd; ++s) dev_axpby(n2, 1.0, H_.p, 1.0, G.p + s * n2, F.p + s * n2, st_);
         PF_HIP(hipStreamSynchronize(st_));
         res.t_fock += now() - tf;
@@ -594,6 +583,10 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         if (dE_ < opt.eps && errD < opt.eps && !fp32_phase) { conv = true; break; }
         Eold = Eel;
     }
+    Dfin_.alloc(n2 * nd);
+    PF_HIP(hipMemcpyAsync(Dfin_.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+    have_fin_ = true;
+    uhf_fin_ = uhf;
     res.iterations = std::min(it, opt.max_iter);
     res.converged = conv ? 1 : 0;
     res.e_elec = Eel;
@@ -610,7 +603,149 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     res.t_total = now() - t0;
 }
 
-template unsigned long long Engine::build_g<double>(const double *, const double *, double *, double *, bool, int);
-template unsigned long long Engine::build_g<float>(const double *, const double *, double *, double *, bool, int);
+// Primitive-pair data variants for the derivative kernels, same layout as
+// PairSet::d_prim: variant 0 weights K by 2 alpha of the first shell; variant
+// 1 is the reversed pair (P - B, K weighted by 2 beta of the second shell);
+// variant 2 the reversed pair with plain K.
+void Engine::setup_grad_pairs() {
+    if (grad_ready_) return;
+    setup_pairs();
+    const auto &sh = basis_.shells;
+    const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
+    for (int c = 0; c < 3; ++c) {
+        const PairSet &ps = pairs_[c];
+        const size_t sz = (size_t)ps.kp * PF_NFIELD * ps.npad;
+        std::vector<double> w(sz, 0.0), rw(sz, 0.0), r(sz, 0.0);
+        for (int u = 0; u < ps.kp; ++u)
+            for (int i = 0; i < ps.npad; ++i) {
+                const size_t o = ((size_t)u * PF_NFIELD + PF_P) * ps.npad + i;
+                w[o] = rw[o] = r[o] = 1.0;
+            }
+        for (int i = 0; i < ps.n; ++i) {
+            const Shell &A = sh[ps.sa[i]], &B = sh[ps.sb[i]];
+            double r2 = 0.0;
+            for (int d = 0; d < 3; ++d) r2 += (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]);
+            int u = 0;
+            for (int x = 0; x < A.np; ++x)
+                for (int y = 0; y < B.np; ++y, ++u) {
+                    const double a = A.exp[x], b = B.exp[y], p = a + b;
+                    const double K = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+                    for (auto *arr : {&w, &rw, &r}) {
+                        double *col = arr->data() + (size_t)u * PF_NFIELD * ps.npad + i;
+                        col[PF_P * ps.npad] = p;
+                        for (int d = 0; d < 3; ++d) {
+                            const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
+                            col[(PF_PX + d) * ps.npad] = P;
+                            col[(PF_AX + d) * ps.npad] = P - (arr == &w ? A.xyz[d] : B.xyz[d]);
+                        }
+                        col[PF_K * ps.npad] = arr == &w ? 2.0 * a * K : (arr == &rw ? 2.0 * b * K : K);
+                    }
+                }
+            for (; u < ps.kp; ++u)
+                for (auto *arr : {&w, &rw, &r}) (*arr)[((size_t)u * PF_NFIELD + PF_K) * ps.npad + i] = 0.0;
+        }
+        gw_[c].upload(w.data(), sz, st_);
+        grw_[c].upload(rw.data(), sz, st_);
+        gr_[c].upload(r.data(), sz, st_);
+    }
+    grad_ready_ = true;
+}
+
+void Engine::gradient(double *grad_out, double tau) {
+    if (!have_fin_) throw Error("gradient: no converged density (run the SCF first)");
+    setup_grad_pairs();
+    const int n = nbf(), nsh = nshell(), na = mol_.natoms;
+    const size_t n2 = (size_t)n * n;
+    const bool uhf = uhf_fin_;
+    const int nd = uhf ? 2 : 1;
+    const double *D0 = Dfin_.p, *D1 = uhf ? Dfin_.p + n2 : nullptr;
+    // F at the final density (full FP64 build, tight screening)
+    DevBuf<double> G(n2 * nd), F(n2 * nd), W(n2), Dt(n2), tmp(n2), grad(3 * (size_t)na);
+    build_g<double>(D0, D1, G.p, uhf ? G.p + n2 : nullptr, false, MODE_DIRECT, 1e-14);
+    for (int s = 0; s < nd; ++s) dev_axpby(n2, 1.0, H_.p, 1.0, G.p + s * n2, F.p + s * n2, st_);
+    // W = 1/2 D F D (RHF) or sum_s D_s F_s D_s (UHF)
+    dev_fill(n2, 0.0, W.p, st_);
+    for (int s = 0; s < nd; ++s) {
+        gemm_f64(n, false, false, 1.0, F.p + s * n2, Dfin_.p + s * n2, 0.0, tmp.p, st_);
+        gemm_f64(n, false, false, uhf ? 1.0 : 0.5, Dfin_.p + s * n2, tmp.p, 1.0, W.p, st_);
+    }
+    if (uhf) dev_axpby(n2, 1.0, D0, 1.0, D1, Dt.p, st_);
+    else PF_HIP(hipMemcpyAsync(Dt.p, D0, sizeof(double) * n2, hipMemcpyDeviceToDevice, st_));
+    dev_fill(3 * (size_t)na, 0.0, grad.p, st_);
+    // ---- one-electron part over ordered shell pairs, by (la, lb)
+    {
+        const auto &sh = basis_.shells;
+        for (int la = 0; la <= 1; ++la)
+            for (int lb = 0; lb <= 1; ++lb) {
+                std::vector<int> pa, pb;
+                for (int a = 0; a < nsh; ++a)
+                    for (int b = 0; b < nsh; ++b)
+                        if (sh[a].l == la && sh[b].l == lb) { pa.push_back(a); pb.push_back(b); }
+                if (pa.empty()) continue;
+                DevBuf<int> dpa, dpb;
+                dpa.upload(pa.data(), pa.size(), st_);
+                dpb.upload(pb.data(), pb.size(), st_);
+                launch_grad1e(la, lb, d_shells_.p, dpa.p, dpb.p, (int)pa.size(), n, na, d_axyz_.p, d_Z_.p,
+                              d_tab_.p, Dt.p, W.p, grad.p, st_);
+                PF_HIP(hipStreamSynchronize(st_));
+            }
+    }
+    // ---- two-electron part
+    dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, Dt.p, D0 != Dt.p ? D0 : nullptr, Dsh64_.p, dmax64_.p,
+                           st_, D1);
+    std::array<DevBuf<double>, 3> gp;
+    for (int c = 0; c < 3; ++c) {
+        gp[c].alloc(std::max<size_t>(6 * (size_t)pairs_[c].n, 1));
+        dev_fill(6 * (size_t)pairs_[c].n, 0.0, gp[c].p, st_);
+    }
+    for (size_t q = 0; q < classes_.size(); ++q) {
+        const int bc = classes_[q].bra, kc = classes_[q].ket;
+        GradArgs g;
+        g.bra = pairs_[bc].view64();
+        g.ket = pairs_[kc].view64();
+        g.bra_w = gw_[bc].p; g.bra_rw = grw_[bc].p; g.bra_r = gr_[bc].p; g.ket_w = gw_[kc].p;
+        g.same = bc == kc;
+        g.tau = tau;
+        g.nbf = n; g.nsh = nsh;
+        g.Dt = Dt.p;
+        g.Dk0 = uhf ? D0 : Dt.p;
+        g.Dk1 = D1;
+        g.Dsh = Dsh64_.p; g.dmax = dmax64_.p;
+        g.gbra = gp[bc].p; g.gket = gp[kc].p;
+        g.tab = d_tab_.p;
+        launch_grad2e_class(pairs_[bc].la, pairs_[bc].lb, pairs_[kc].la, pairs_[kc].lb, uhf ? 2 : 1, g, st_);
+    }
+    std::vector<double> hg(3 * (size_t)na);
+    grad.download(hg.data(), hg.size(), st_);
+    std::array<std::vector<double>, 3> hp;
+    for (int c = 0; c < 3; ++c) {
+        hp[c].resize(6 * (size_t)pairs_[c].n);
+        gp[c].download(hp[c].data(), hp[c].size(), st_);
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+    const auto &sh = basis_.shells;
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < pairs_[c].n; ++i)
+            for (int x = 0; x < 3; ++x) {
+                hg[3 * sh[pairs_[c].sa[i]].atom + x] += hp[c][6 * (size_t)i + x];
+                hg[3 * sh[pairs_[c].sb[i]].atom + x] += hp[c][6 * (size_t)i + 3 + x];
+            }
+    // nuclear repulsion: d/dR_A sum_{B != A} Z_A Z_B / R_AB
+    for (int A = 0; A < na; ++A)
+        for (int B = 0; B < na; ++B) {
+            if (A == B) continue;
+            double d[3], r2 = 0.0;
+            for (int x = 0; x < 3; ++x) { d[x] = mol_.xyz[3 * A + x] - mol_.xyz[3 * B + x]; r2 += d[x] * d[x]; }
+            const double r = std::sqrt(r2);
+            if (r < 1e-10) continue;
+            for (int x = 0; x < 3; ++x) hg[3 * A + x] -= double(mol_.Z[A]) * mol_.Z[B] * d[x] / (r2 * r);
+        }
+    std::copy(hg.begin(), hg.end(), grad_out);
+}
+
+template unsigned long long Engine::build_g<double>(const double *, const double *, double *, double *, bool, int,
+                                                    double);
+template unsigned long long Engine::build_g<float>(const double *, const double *, double *, double *, bool, int,
+                                                   double);
 
 }  // namespace pf

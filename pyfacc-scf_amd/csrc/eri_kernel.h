@@ -172,13 +172,13 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
                     R s = R(0);
 #pragma unroll
                     for (int ab = 0; ab < NA * NB; ++ab) s += eri[ab * NC * ND + ic * ND + id] * dab[ab];
-                    atomicAdd(&g.J[(size_t)(oc + ic) * nbf + od + id], deg2 * s);
+                    atomicAdd(&g.J[(size_t)(oc + ic) * nbf + od + id], double(deg2 * s));
                 }
             // ---- exchange, one pass per density
 #pragma unroll
             for (int kk = 0; kk < NK; ++kk) {
                 const R *Dk = kk == 0 ? g.Dk0 : g.Dk1;
-                R *Kout = kk == 0 ? g.K0 : g.K1;
+                double *Kout = kk == 0 ? g.K0 : g.K1;
                 R dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
 #pragma unroll
                 for (int ib = 0; ib < NB; ++ib) {
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
                         for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
                             for (int id = 0; id < ND; ++id) s += ERI_(ia, ib, ic, id) * dbd[ib * ND + id];
-                        atomicAdd(&Kout[(size_t)(oa + ia) * nbf + oc + ic], deg * s);
+                        atomicAdd(&Kout[(size_t)(oa + ia) * nbf + oc + ic], double(deg * s));
                     }
 #pragma unroll
                 for (int ia = 0; ia < NA; ++ia)
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
                         for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
                             for (int ic = 0; ic < NC; ++ic) s += ERI_(ia, ib, ic, id) * dbc[ib * NC + ic];
-                        atomicAdd(&Kout[(size_t)(oa + ia) * nbf + od + id], deg * s);
+                        atomicAdd(&Kout[(size_t)(oa + ia) * nbf + od + id], double(deg * s));
                     }
 #pragma unroll
                 for (int ib = 0; ib < NB; ++ib)
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
                         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                             for (int id = 0; id < ND; ++id) s += ERI_(ia, ib, ic, id) * dad[ia * ND + id];
-                        atomicAdd(&Kout[(size_t)(ob + ib) * nbf + oc + ic], deg * s);
+                        atomicAdd(&Kout[(size_t)(ob + ib) * nbf + oc + ic], double(deg * s));
                     }
 #pragma unroll
                 for (int ib = 0; ib < NB; ++ib)
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
                         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                             for (int ic = 0; ic < NC; ++ic) s += ERI_(ia, ib, ic, id) * dac[ia * NC + ic];
-                        atomicAdd(&Kout[(size_t)(ob + ib) * nbf + od + id], deg * s);
+                        atomicAdd(&Kout[(size_t)(ob + ib) * nbf + od + id], double(deg * s));
                     }
 #undef ERI_
             }
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
     if constexpr (MODE == MODE_DIRECT || MODE == MODE_LOAD) {
 #pragma unroll
         for (int ab = 0; ab < NA * NB; ++ab) {
-            const R s = wave_sum(jab[ab]);
+            const double s = wave_sum(double(jab[ab]));
             if (lane == 0) atomicAdd(&g.J[(size_t)(oa + ab / NB) * nbf + ob + ab % NB], s);
         }
     }

@@ -1,0 +1,6 @@
+// Instantiates the two-electron gradient kernel of quartet class (ps|ps).
+#include "grad_kernel.h"
+
+namespace pf {
+template void launch_grad2e<1, 0, 1, 0>(int, const GradArgs &, hipStream_t);
+}  // namespace pf

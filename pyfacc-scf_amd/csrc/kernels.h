@@ -41,7 +41,9 @@ struct JKArgs {
     const R *Dk1 = nullptr;
     const R *Dsh = nullptr;  // [nsh*nsh] max |D| per shell block (screening), nullptr = Schwarz only
     const R *dmax = nullptr; // device scalar: 2 * max(Dsh)
-    R *J = nullptr, *K0 = nullptr, *K1 = nullptr;  // non-symmetric accumulators J', K'
+    // non-symmetric accumulators J', K': FP64 in every precision (FP32 builds
+    // form each quartet's contribution in FP32 and add it in FP64)
+    double *J = nullptr, *K0 = nullptr, *K1 = nullptr;
     const R *tab = nullptr;  // Boys table, stride BOYS_MMAX+1
     // stored-integral mode
     R *store = nullptr;
@@ -63,10 +65,22 @@ void launch_jk_class(int la, int lb, int lc, int ld, int nk, int mode, const JKA
 template <typename R>
 void launch_schwarz(int la, int lb, const PairView<R> &pv, const R *tab, R *Q, hipStream_t st);
 
+// ---------------------------------------------------------------- gradient
+struct GradArgs;
+void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int *pb, int npairs, int nbf, int natoms,
+                   const double *axyz, const int *Z, const double *tab, const double *D, const double *W,
+                   double *grad, hipStream_t st);
+void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, hipStream_t st);
+
 // ---------------------------------------------------------------- dense FP64
 // C = alpha op(A) op(B) + beta C, row-major n x n (lda = ldb = ldc = n), MFMA f64.
 void gemm_f64(int n, bool ta, bool tb, double alpha, const double *A, const double *B, double beta, double *C,
               hipStream_t st);
+// General variant: C = alpha op(A) op(B) + beta Cin; alpha_p / beta_p (device
+// pointers) override alpha / beta when non-null.
+void gemm_f64_ex(int m, int n, int k, bool ta, bool tb, double alpha, const double *alpha_p, const double *A, int lda,
+                 const double *B, int ldb, double beta, const double *beta_p, const double *Cin, double *C, int ldc,
+                 hipStream_t st);
 // General m x n x k row-major variant.
 void gemm_f64_mnk(int m, int n, int k, bool ta, bool tb, double alpha, const double *A, int lda,
                   const double *B, int ldb, double beta, double *C, int ldc, hipStream_t st);

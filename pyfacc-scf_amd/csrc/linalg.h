@@ -22,7 +22,7 @@ void dev_to_float(int64_t n, double scale, const double *x, const double *y, flo
 void dev_antisym(int n, const double *M, double *E, hipStream_t st);
 template <typename R>
 void dev_shell_dmax(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const double *D0, const double *D1, R *Dsh,
-                    R *dmax, hipStream_t st);
+                    R *dmax, hipStream_t st, const double *D2 = nullptr);
 // Symmetric eigensolver: A (n x n, row-major) is read only; w ascending, V
 // columns are eigenvectors.  Synchronises the stream.
 void jacobi_eigh(int n, const double *A, double *w, double *V, hipStream_t st, int max_sweeps = 30,

@@ -26,83 +26,80 @@ namespace pf {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ GEMM
-// Row-major: element (r, c) of A is A[r*lda + c].  op(A) is m x k.
-__global__ void __launch_bounds__(64) gemm_f64_kernel(int m, int n, int k, int ta, int tb, double alpha,
-                                                       const double *__restrict__ A, int lda,
-                                                       const double *__restrict__ B, int ldb, double beta,
-                                                       double *__restrict__ C, int ldc) {
-    __shared__ double As[32][17];  // [row][kk]
-    __shared__ double Bs[16][33];  // [kk][col]
-    const int lane = threadIdx.x;
+// Row-major: element (r, c) of A is A[r*lda + c]; op(A) is m x k.
+// C = alpha op(A) op(B) + beta Cin (Cin may alias C).  alpha_p / beta_p, when
+// non-null, are read from device memory (device-side branch selection).
+// One 256-thread workgroup per 32x32 tile of C: the four wavefronts split K
+// (interleaved steps of 4), each accumulating the 2x2 grid of 16x16 MFMA
+// blocks from operands loaded straight from L2, then the four partial tiles
+// are summed through LDS.  For n = 448 this is 196 tiles x 4 wavefronts.
+__device__ __forceinline__ double ld_op(const double *__restrict__ M, int ld, int r, int c, int nr, int nc, int t) {
+    // element (r, c) of op(M) (op(M) is nr x nc); t: M stored transposed
+    if (r >= nr || c >= nc) return 0.0;
+    return t ? M[(size_t)c * ld + r] : M[(size_t)r * ld + c];
+}
+
+__global__ void __launch_bounds__(256) gemm_f64_kernel(int m, int n, int k, int ta, int tb, double alpha,
+                                                        const double *__restrict__ alpha_p,
+                                                        const double *__restrict__ A, int lda,
+                                                        const double *__restrict__ B, int ldb, double beta,
+                                                        const double *__restrict__ beta_p, const double *Cin,
+                                                        double *C, int ldc) {
+    __shared__ double red[4][16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int i = lane & 15, kq = lane >> 4;
     double4_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0, 0, 0, 0};
-
-    for (int k0 = 0; k0 < k; k0 += 16) {
-        // stage A tile (32 x 16) and B tile (16 x 32): 8 elements each per lane
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int e = lane + 64 * t;  // 0..511
-            {
-                int rr, kk;
-                if (!ta) { rr = e / 16; kk = e % 16; }  // consecutive lanes walk k (contiguous in row-major A)
-                else { kk = e / 32; rr = e % 32; }      // A^T: consecutive lanes walk rows of op(A) (contiguous)
-                const int gr = r0 + rr, gk = k0 + kk;
-                double v = 0.0;
-                if (gr < m && gk < k) v = ta ? A[(size_t)gk * lda + gr] : A[(size_t)gr * lda + gk];
-                As[rr][kk] = v;
-            }
-            {
-                int kk, cc;
-                if (!tb) { kk = e / 32; cc = e % 32; }
-                else { cc = e / 16; kk = e % 16; }
-                const int gk = k0 + kk, gc = c0 + cc;
-                double v = 0.0;
-                if (gk < k && gc < n) v = tb ? B[(size_t)gc * ldb + gk] : B[(size_t)gk * ldb + gc];
-                Bs[kk][cc] = v;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < 16; ks += 4) {
-            const int i = lane & 15, kk = ks + (lane >> 4);
-            const double a0 = As[i][kk], a1 = As[16 + i][kk];
-            const double b0 = Bs[kk][i], b1 = Bs[kk][16 + i];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        }
-        __syncthreads();
+#pragma unroll 4
+    for (int kb = 4 * w; kb < k; kb += 16) {
+        const int kk = kb + kq;
+        // v_mfma_f64_16x16x4f64 operands: A lane l = (row l & 15, k l >> 4); B lane l = (k l >> 4, col l & 15)
+        const double a0 = ld_op(A, lda, r0 + i, kk, m, k, ta), a1 = ld_op(A, lda, r0 + 16 + i, kk, m, k, ta);
+        const double b0 = ld_op(B, ldb, kk, c0 + i, k, n, tb), b1 = ld_op(B, ldb, kk, c0 + 16 + i, k, n, tb);
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
-    // D layout of v_mfma_f64_16x16x4f64: register r of lane l holds row
-    // (l >> 4) + 4 r, column l & 15 (A: lane l holds row l & 15, k l >> 4;
-    // B: lane l holds k l >> 4, column l & 15)
-    const int col = lane & 15, rb = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gr = r0 + 16 * a + rb + 4 * r, gc = c0 + 16 * b + col;
-                if (gr < m && gc < n) {
-                    double *dst = C + (size_t)gr * ldc + gc;
-                    const double v = alpha * acc[a][b][r];
-                    *dst = beta == 0.0 ? v : v + beta * *dst;
-                }
-            }
+            for (int r = 0; r < 4; ++r) red[w][(a * 2 + b) * 4 + r][lane] = acc[a][b][r];
+    __syncthreads();
+    const double al = alpha_p ? *alpha_p : alpha, be = beta_p ? *beta_p : beta;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int idx = threadIdx.x + 256 * q;  // 0..1023 = (reg, lane)
+        const int reg = idx >> 6, ln = idx & 63;
+        const double v = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+        // D layout: register r of lane l holds row (l >> 4) + 4 r, column l & 15
+        const int a = reg >> 3, b = (reg >> 2) & 1, r = reg & 3;
+        const int gr = r0 + 16 * a + (ln >> 4) + 4 * r, gc = c0 + 16 * b + (ln & 15);
+        if (gr < m && gc < n) {
+            const size_t o = (size_t)gr * ldc + gc;
+            C[o] = be == 0.0 ? al * v : al * v + be * Cin[o];
+        }
+    }
+}
+
+void gemm_f64_ex(int m, int n, int k, bool ta, bool tb, double alpha, const double *alpha_p, const double *A, int lda,
+                 const double *B, int ldb, double beta, const double *beta_p, const double *Cin, double *C, int ldc,
+                 hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    dim3 grid((n + 31) / 32, (m + 31) / 32);
+    gemm_f64_kernel<<<grid, 256, 0, st>>>(m, n, k, ta, tb, alpha, alpha_p, A, lda, B, ldb, beta, beta_p, Cin, C, ldc);
+    PF_HIP(hipGetLastError());
 }
 
 void gemm_f64_mnk(int m, int n, int k, bool ta, bool tb, double alpha, const double *A, int lda, const double *B,
                   int ldb, double beta, double *C, int ldc, hipStream_t st) {
-    if (m <= 0 || n <= 0) return;
-    dim3 grid((n + 31) / 32, (m + 31) / 32);
-    gemm_f64_kernel<<<grid, 64, 0, st>>>(m, n, k, ta, tb, alpha, A, lda, B, ldb, beta, C, ldc);
-    PF_HIP(hipGetLastError());
+    gemm_f64_ex(m, n, k, ta, tb, alpha, nullptr, A, lda, B, ldb, beta, nullptr, C, C, ldc, st);
 }
 
 void gemm_f64(int n, bool ta, bool tb, double alpha, const double *A, const double *B, double beta, double *C,
@@ -259,7 +256,7 @@ void dev_antisym(int n, const double *M, double *E, hipStream_t st) {
 // mu in a, nu in b, over up to two densities; then dmax = 2 max(Dsh).
 template <typename R>
 __global__ void shell_dmax_kernel(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const double *D0,
-                                  const double *D1, R *Dsh) {
+                                  const double *D1, const double *D2, R *Dsh) {
     const int a = blockIdx.y, b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nsh) return;
     double m = 0.0;
@@ -268,6 +265,7 @@ __global__ void shell_dmax_kernel(int nsh, int nbf, const int *sh_bf, const int 
             const size_t ij = (size_t)(sh_bf[a] + i) * nbf + sh_bf[b] + j;
             m = fmax(m, fabs(D0[ij]));
             if (D1) m = fmax(m, fabs(D1[ij]));
+            if (D2) m = fmax(m, fabs(D2[ij]));
         }
     Dsh[(size_t)a * nsh + b] = R(m);
 }
@@ -287,16 +285,16 @@ __global__ void __launch_bounds__(1024) max_kernel(int64_t n, const R *x, R scal
 }
 template <typename R>
 void dev_shell_dmax(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const double *D0, const double *D1, R *Dsh,
-                    R *dmax, hipStream_t st) {
+                    R *dmax, hipStream_t st, const double *D2) {
     dim3 grid((nsh + 63) / 64, nsh);
-    shell_dmax_kernel<R><<<grid, 64, 0, st>>>(nsh, nbf, sh_bf, sh_nf, D0, D1, Dsh);
+    shell_dmax_kernel<R><<<grid, 64, 0, st>>>(nsh, nbf, sh_bf, sh_nf, D0, D1, D2, Dsh);
     max_kernel<R><<<1, 1024, 0, st>>>((int64_t)nsh * nsh, Dsh, R(2), dmax);
     PF_HIP(hipGetLastError());
 }
 template void dev_shell_dmax<double>(int, int, const int *, const int *, const double *, const double *, double *,
-                                     double *, hipStream_t);
+                                     double *, hipStream_t, const double *);
 template void dev_shell_dmax<float>(int, int, const int *, const int *, const double *, const double *, float *,
-                                    float *, hipStream_t);
+                                    float *, hipStream_t, const double *);
 
 // ------------------------------------------------------------------ Jacobi
 // Round-robin (circle method) pairing over N = even padded size: player N-1
@@ -477,40 +475,56 @@ __global__ void sp2_init_kernel(int n, const double *F, const double *bounds, do
     const double s = 1.0 / (hi - lo);
     X[(size_t)i * n + j] = ((i == j ? hi : 0.0) - F[(size_t)i * n + j]) * s;
 }
-// state[0] = tr X, state[1] = tr X^2 (filled by the trace kernels); state[2] = done flag;
-// choose X <- X^2 if |tr X^2 - nocc| < |2 tr X - tr X^2 - nocc| else 2X - X^2.
-__global__ void sp2_update_kernel(int n, double nocc, const double *X2, double *X, double *state) {
-    if (state[2] != 0.0) return;
-    const double t1 = state[0], t2 = state[1];
-    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const size_t ij = (size_t)i * n + j;
-    X[ij] = square ? X2[ij] : 2.0 * X[ij] - X2[ij];
-}
-__global__ void sp2_check_kernel(double *state, int iter, double tol) {
-    // idempotency error tr X - tr X^2 >= 0; stop when it no longer shrinks below tol
-    if (state[2] != 0.0) return;
-    const double err = fabs(state[0] - state[1]);
-    if (iter > 2 && (err < tol || (err < 1e-8 && err >= state[3]))) state[2] = 1.0;
-    state[3] = err;
+// One block: t1 = tr X, t2 = sum X_ij^2 (= tr X^2 for symmetric X).  Chooses
+// the next step X <- X^2 (alpha 1, beta 0) when |t2 - nocc| < |2 t1 - t2 - nocc|,
+// else X <- 2X - X^2 (alpha -1, beta 2); once the idempotency error t1 - t2 is
+// below tol (or stops shrinking below 1e-8) the step becomes X <- X
+// (alpha 0, beta 1) and state[2] flags completion.
+// state: 0 t1, 1 t2, 2 done, 3 previous error, 4 alpha, 5 beta, 6/7 bounds
+__global__ void __launch_bounds__(1024) sp2_step_kernel(int n, double nocc, const double *X, double *state,
+                                                         int iter, double tol) {
+    __shared__ double p1[16], p2[16];
+    double t1 = 0.0, t2 = 0.0;
+    const size_t n2 = (size_t)n * n;
+    for (size_t e = threadIdx.x; e < n2; e += blockDim.x) {
+        const double v = X[e];
+        t2 += v * v;
+        if (e % (n + 1) == 0) t1 += v;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        t1 += __shfl_xor(t1, o, 64);
+        t2 += __shfl_xor(t2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) { p1[threadIdx.x >> 6] = t1; p2[threadIdx.x >> 6] = t2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        t1 = 0.0; t2 = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { t1 += p1[w]; t2 += p2[w]; }
+        state[0] = t1;
+        state[1] = t2;
+        const double err = fabs(t1 - t2);
+        if (state[2] == 0.0 && iter > 2 && (err < tol || (err < 1e-8 && err >= state[3]))) state[2] = 1.0;
+        state[3] = err;
+        if (state[2] != 0.0) { state[4] = 0.0; state[5] = 1.0; }
+        else if (fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc)) { state[4] = 1.0; state[5] = 0.0; }
+        else { state[4] = -1.0; state[5] = 2.0; }
+    }
 }
 
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
                 int max_iter) {
     dim3 g2((n + 127) / 128, n);
-    gershgorin_kernel<<<1, 1024, 0, st>>>(n, Fp, state + 4);
-    sp2_init_kernel<<<g2, 128, 0, st>>>(n, Fp, state + 4, X);
-    dev_fill(4, 0.0, state, st);
+    gershgorin_kernel<<<1, 1024, 0, st>>>(n, Fp, state + 6);
+    sp2_init_kernel<<<g2, 128, 0, st>>>(n, Fp, state + 6, X);
+    dev_fill(3, 0.0, state, st);
     dev_fill(1, 1e300, state + 3, st);
     const double tol = 1e-13 * std::max(1.0, nocc);
+    double *cur = X, *nxt = work;
     int it = 0;
     for (; it < max_iter; ++it) {
-        gemm_f64(n, false, false, 1.0, X, X, 0.0, work, st);
-        dev_trace(n, X, state + 0, st);
-        dev_trace(n, work, state + 1, st);
-        sp2_check_kernel<<<1, 1, 0, st>>>(state, it, tol);
-        sp2_update_kernel<<<g2, 128, 0, st>>>(n, nocc, work, X, state);
+        sp2_step_kernel<<<1, 1024, 0, st>>>(n, nocc, cur, state, it, tol);
+        gemm_f64_ex(n, n, n, false, false, 0.0, state + 4, cur, n, cur, n, 0.0, state + 5, cur, nxt, n, st);
+        std::swap(cur, nxt);
         if ((it & 7) == 7) {
             double done = 0.0;
             PF_HIP(hipMemcpyAsync(&done, state + 2, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -518,8 +532,13 @@ int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, d
             if (done != 0.0) break;
         }
     }
-    symmetrize_kernel<<<g2, 128, 0, st>>>(n, X, work);
-    PF_HIP(hipMemcpyAsync(X, work, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
+    // symmetrise (X + X^T)/2 into X
+    if (cur == X) {
+        symmetrize_kernel<<<g2, 128, 0, st>>>(n, X, work);
+        PF_HIP(hipMemcpyAsync(X, work, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
+    } else {
+        symmetrize_kernel<<<g2, 128, 0, st>>>(n, work, X);
+    }
     PF_HIP(hipGetLastError());
     return it;
 }

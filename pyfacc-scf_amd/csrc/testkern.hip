@@ -36,3 +36,32 @@ void boys_device(int nmax, int npts, const double *x, double *F, hipStream_t st)
 }
 
 }  // namespace pf
+
+namespace pf {
+
+__global__ void copy_kernel(size_t n, const double4 *__restrict__ src, double4 *__restrict__ dst) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Device-to-device copy of `bytes` (rounded down to 32 B) with a plain
+// grid-stride kernel; returns the kernel time in ms (HIP events).
+double device_copy(size_t bytes, hipStream_t st) {
+    const size_t n = bytes / sizeof(double4);
+    DevBuf<double4> a(n), b(n);
+    PF_HIP(hipMemsetAsync(a.p, 0, n * sizeof(double4), st));
+    hipEvent_t e0, e1;
+    PF_HIP(hipEventCreate(&e0));
+    PF_HIP(hipEventCreate(&e1));
+    PF_HIP(hipEventRecord(e0, st));
+    copy_kernel<<<2048, 256, 0, st>>>(n, a.p, b.p);
+    PF_HIP(hipEventRecord(e1, st));
+    PF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+}  // namespace pf

@@ -20,7 +20,8 @@ class Options(ctypes.Structure):
     _fields_ = [("eps", ctypes.c_double), ("max_iter", ctypes.c_int), ("precision", ctypes.c_int),
                 ("incremental", ctypes.c_int), ("eri_mode", ctypes.c_int), ("diis", ctypes.c_int),
                 ("diag", ctypes.c_int), ("renormalize", ctypes.c_int), ("schwarz", ctypes.c_double),
-                ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int)]
+                ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int), ("schwarz_inc", ctypes.c_double),
+                ("rebuild_every", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -52,10 +53,13 @@ _SIGS = {
     "pf_jk": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pd, _pd, _pd, _i, _i, _d]),
     "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
     "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i)]),
+    "pf_scf_gradient": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd,
+                             _pd]),
     "pf_boys": (_i, [_i, _i, _pd, _pd]),
     "pf_eigh": (_i, [_i, _pd, _pd, _pd]),
     "pf_projector": (_i, [_i, _pd, _d, _pd]),
     "pf_gemm": (_i, [_i, _i, _i, _i, _i, _d, _pd, _pd, _d, _pd]),
+    "pf_device_copy": (_i, [ctypes.c_ulonglong, _pd]),
 }
 
 
@@ -198,6 +202,13 @@ def scf_profile():
     load_library().pf_scf_profile(q, _ptr(ms), n.ctypes.data_as(_P(_i)), cl.ctypes.data_as(_P(_i)))
     return (np.array(list(q), dtype=np.int64).reshape(2, 6), ms.reshape(2, 6), n.reshape(2, 6),
             cl.reshape(6, 4))
+
+
+def device_copy(nbytes):
+    """Time a plain device-to-device copy of nbytes (ms)."""
+    ms = ctypes.c_double(0.0)
+    _check(load_library().pf_device_copy(int(nbytes), ctypes.byref(ms)))
+    return ms.value
 
 
 def boys(nmax, x):

@@ -1,0 +1,56 @@
+"""Nuclear gradients behind torch.autograd.
+
+``scf_energy(coords, species, nalpha, nbeta, ...)`` returns the SCF total
+energy as a differentiable torch scalar of the (natoms, 3) coordinate tensor
+(bohr).  Forward runs ``pf_scf_gradient`` (SCF + analytic gradient on the GPU
+through the C-ABI) and keeps dE/dR; backward scales it by the incoming
+gradient.  PyTorch is used only for the autograd bookkeeping: all integrals,
+the SCF and the derivative contractions run in the HIP library.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+
+def scf_gradient(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, **kw):
+    """(energy, gradient (natoms, 3), result dict) from pf_scf_gradient."""
+    opt = options if options is not None else _native.default_options(**kw)
+    m = _native._Mol(species, coords_bohr)
+    g = np.zeros((m.n, 3))
+    r = _native.Result()
+    _native._check(_native.load_library().pf_scf_gradient(
+        *m.args(), basis.encode(), int(nalpha), int(nbeta), ctypes.byref(opt), ctypes.byref(r),
+        g.ctypes.data_as(_native._pd), None))
+    return r.energy, g, r.as_dict()
+
+
+def _function():
+    import torch
+
+    class SCFEnergy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, coords, species, nalpha, nbeta, basis, options):
+            e, g, _ = scf_gradient(species, coords.detach().cpu().double().numpy(), nalpha, nbeta, basis, options)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=coords.dtype, device=coords.device))
+            return coords.new_tensor(e)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            return grad_out * g, None, None, None, None, None
+
+    return SCFEnergy
+
+
+_FN = None
+
+
+def scf_energy(coords, species, nalpha, nbeta, basis="sto-3g", options=None, **kw):
+    """Differentiable SCF energy of a (natoms, 3) torch tensor in bohr."""
+    global _FN
+    if _FN is None:
+        _FN = _function()
+    opt = options if options is not None else _native.default_options(**kw)
+    return _FN.apply(coords, list(species), int(nalpha), int(nbeta), basis, opt)
