@@ -133,9 +133,13 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
  * converged density: grad[natoms*3] in Eh/bohr (one-electron, two-electron
  * and nuclear-repulsion terms; W = 1/2 D F D, RHF, or sum_s D_s F_s D_s, UHF).
  * The reference has no gradients; this is the north star's differentiation
- * extension, bound to torch.autograd by pyfacc.grad.  D_out may be NULL. */
+ * extension, bound to torch.autograd by pyfacc.grad.  D_out may be NULL;
+ * terms (may be NULL) receives 5 x natoms x 3 contributions: nuclear
+ * repulsion, kinetic (D dT), nuclear attraction (D dV), overlap (-W dS),
+ * two-electron. */
 int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
-                    int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *grad, double *D_out);
+                    int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *grad, double *D_out,
+                    double *terms);
 
 /* Fock-build profile of the last pf_scf call on this thread, summed over its
  * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,

@@ -500,41 +500,52 @@ void or_eri_packed(int nsh, const int *l, const int *np, const double *ctr, cons
     free(off);
 }
 
-/* J[D] and K[D] from packed unique ERIs, textbook 8-fold scatter.
- * J_mn = sum_ls (mn|ls) D_ls ; K_mn = sum_ls (ml|ns) D_ls.  D symmetric. */
+/* J[D] and K[D] from packed unique ERIs: each unique (ij|kl) is scaled by
+ * 1/2 for each coincidence i=j, k=l, ij=kl and scattered into non-symmetric
+ * J', K' (J'_ij += 2v D_kl, J'_kl += 2v D_ij, K'_ik += v D_jl, K'_il += v D_jk,
+ * K'_jk += v D_il, K'_jl += v D_ik), then J = J' + J'^T, K = K' + K'^T.
+ * J_mn = sum_ls (mn|ls) D_ls ; K_mn = sum_ls (ml|ns) D_ls.  D symmetric.
+ * Pinned against the full-tensor contraction in the tests. */
 void or_jk_packed(int nbf, const double *packed, int ndens, const double *D, double *J, double *K)
 {
     size_t n2 = (size_t)nbf * nbf;
     memset(J, 0, sizeof(double) * n2 * ndens);
     memset(K, 0, sizeof(double) * n2 * ndens);
-    for (int i = 0; i < nbf; ++i)
-        for (int j = 0; j <= i; ++j)
-            for (int k = 0; k <= i; ++k)
-                for (int l = 0; l <= (k == i ? j : k); ++l) {
-                    double v = packed[quad_idx(i, j, k, l)];
-                    if (v == 0.0) continue;
-                    /* enumerate the distinct index permutations of (ij|kl) */
-                    int perm[8][4] = {{i, j, k, l}, {j, i, k, l}, {i, j, l, k}, {j, i, l, k},
-                                      {k, l, i, j}, {l, k, i, j}, {k, l, j, i}, {l, k, j, i}};
-                    int np_ = 0, uniq[8][4];
-                    for (int p = 0; p < 8; ++p) {
-                        int dup = 0;
-                        for (int q = 0; q < np_; ++q)
-                            if (uniq[q][0] == perm[p][0] && uniq[q][1] == perm[p][1] && uniq[q][2] == perm[p][2] &&
-                                uniq[q][3] == perm[p][3]) { dup = 1; break; }
-                        if (!dup) { memcpy(uniq[np_], perm[p], sizeof perm[p]); ++np_; }
-                    }
-                    for (int dd = 0; dd < ndens; ++dd) {
-                        const double *Dd = D + dd * n2;
-                        double *Jd = J + dd * n2, *Kd = K + dd * n2;
-                        for (int p = 0; p < np_; ++p) {
-                            int m = uniq[p][0], n = uniq[p][1], a = uniq[p][2], s = uniq[p][3];
-                            Jd[m * nbf + n] += v * Dd[a * nbf + s];
-                            /* (m n|a s) = (m l'|n' s') with K_{m a} += (m n|a s) D_{n s} */
-                            Kd[m * nbf + a] += v * Dd[n * nbf + s];
+    for (int dd = 0; dd < ndens; ++dd) {
+        const double *Dd = D + dd * n2;
+        double *Jd = J + dd * n2, *Kd = K + dd * n2;
+#pragma omp parallel
+        {
+            double *Jl = (double *)calloc(n2, sizeof(double)), *Kl = (double *)calloc(n2, sizeof(double));
+#pragma omp for schedule(dynamic, 8)
+            for (int i = 0; i < nbf; ++i)
+                for (int j = 0; j <= i; ++j)
+                    for (int k = 0; k <= i; ++k)
+                        for (int l = 0; l <= (k == i ? j : k); ++l) {
+                            double v = packed[quad_idx(i, j, k, l)];
+                            if (v == 0.0) continue;
+                            if (i == j) v *= 0.5;
+                            if (k == l) v *= 0.5;
+                            if (i == k && j == l) v *= 0.5;
+                            Jl[i * nbf + j] += 2.0 * v * Dd[k * nbf + l];
+                            Jl[k * nbf + l] += 2.0 * v * Dd[i * nbf + j];
+                            Kl[i * nbf + k] += v * Dd[j * nbf + l];
+                            Kl[i * nbf + l] += v * Dd[j * nbf + k];
+                            Kl[j * nbf + k] += v * Dd[i * nbf + l];
+                            Kl[j * nbf + l] += v * Dd[i * nbf + k];
                         }
-                    }
-                }
+#pragma omp critical
+            for (size_t q = 0; q < n2; ++q) { Jd[q] += Jl[q]; Kd[q] += Kl[q]; }
+            free(Jl);
+            free(Kl);
+        }
+        for (int i = 0; i < nbf; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double sj = Jd[i * nbf + j] + Jd[j * nbf + i], sk = Kd[i * nbf + j] + Kd[j * nbf + i];
+                Jd[i * nbf + j] = Jd[j * nbf + i] = sj;
+                Kd[i * nbf + j] = Kd[j * nbf + i] = sk;
+            }
+    }
 }
 
 /* Schwarz bound per shell pair: Q_ab = sqrt(max |(ab|ab)|), row-major nsh x nsh. */

@@ -175,7 +175,8 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
 }
 
 int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
-                    int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *grad, double *D_out) {
+                    int nalpha, int nbeta, const pf_options *opt, pf_result *res, double *grad, double *D_out,
+                    double *terms) {
     return guarded([&] {
         pf_options o;
         if (opt) o = *opt;
@@ -187,7 +188,7 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
                    o.schwarz > 0 ? o.schwarz : 1e-14);
         pf_result r;
         eng.scf(nalpha, nbeta, o, r, D_out, nullptr);
-        eng.gradient(grad, 1e-13);
+        eng.gradient(grad, 1e-13, terms);
         if (res) *res = r;
     });
 }

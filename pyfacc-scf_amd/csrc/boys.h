@@ -55,14 +55,16 @@ struct BoysTraits<float> {
     __device__ static float rsqrt_pi_over(float x) { return sqrtf(3.14159265358979f / x); }
 };
 
-// Device: F[0..M] at x.  tab points to the table in R precision.
-template <int M, typename R, int STRIDE = BOYS_MMAX + 1>
+// Device: F[0..M] at x.  tab points to the table in R precision: row k holds
+// orders BASE .. BASE+STRIDE-1 of grid point k (the full table: BASE 0,
+// STRIDE BOYS_MMAX+1; a workgroup's LDS copy: BASE M, STRIDE terms).
+template <int M, typename R, int STRIDE = BOYS_MMAX + 1, int BASE = 0>
 __device__ __forceinline__ void boys_eval(R x, const R *__restrict__ tab, R *F) {
-    static_assert(M + BOYS_TAYLOR <= STRIDE, "Boys table too short");
+    static_assert(M >= BASE && M - BASE + BoysTraits<R>::terms <= STRIDE, "Boys table too short");
     if (x < R(BOYS_XMAX)) {
         const int k = int(x * R(1.0 / BOYS_H) + R(0.5));
         const R d = R(k) * R(BOYS_H) - x;
-        const R *t = tab + k * STRIDE + M;
+        const R *t = tab + k * STRIDE + (M - BASE);
         constexpr int NT = BoysTraits<R>::terms;
         R acc = t[NT - 1];
 #pragma unroll

@@ -58,7 +58,9 @@ class Engine {
     void scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out);
     // Nuclear gradient (natoms x 3, Eh/bohr) at the density of the last scf()
     // call; two-electron quartets screened at tau on Q Q max|Gamma|.
-    void gradient(double *grad_out, double tau);
+    // terms (optional, 5 x natoms x 3): nuclear repulsion, kinetic (D dT),
+    // nuclear attraction (D dV), overlap (-W dS), two-electron.
+    void gradient(double *grad_out, double tau, double *terms = nullptr);
 
     // Fock-build statistics accumulated over an SCF, index precision*6 + class
     // (precision 0 = FP64, 1 = FP32): quartets, kernel milliseconds (HIP events

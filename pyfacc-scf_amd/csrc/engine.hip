@@ -651,7 +651,7 @@ void Engine::setup_grad_pairs() {
     grad_ready_ = true;
 }
 
-void Engine::gradient(double *grad_out, double tau) {
+void Engine::gradient(double *grad_out, double tau, double *terms) {
     if (!have_fin_) throw Error("gradient: no converged density (run the SCF first)");
     setup_grad_pairs();
     const int n = nbf(), nsh = nshell(), na = mol_.natoms;
@@ -660,7 +660,7 @@ void Engine::gradient(double *grad_out, double tau) {
     const int nd = uhf ? 2 : 1;
     const double *D0 = Dfin_.p, *D1 = uhf ? Dfin_.p + n2 : nullptr;
     // F at the final density (full FP64 build, tight screening)
-    DevBuf<double> G(n2 * nd), F(n2 * nd), W(n2), Dt(n2), tmp(n2), grad(3 * (size_t)na);
+    DevBuf<double> G(n2 * nd), F(n2 * nd), W(n2), Dt(n2), tmp(n2), grad(9 * (size_t)na);
     build_g<double>(D0, D1, G.p, uhf ? G.p + n2 : nullptr, false, MODE_DIRECT, 1e-14);
     for (int s = 0; s < nd; ++s) dev_axpby(n2, 1.0, H_.p, 1.0, G.p + s * n2, F.p + s * n2, st_);
     // W = 1/2 D F D (RHF) or sum_s D_s F_s D_s (UHF)
@@ -671,7 +671,7 @@ void Engine::gradient(double *grad_out, double tau) {
     }
     if (uhf) dev_axpby(n2, 1.0, D0, 1.0, D1, Dt.p, st_);
     else PF_HIP(hipMemcpyAsync(Dt.p, D0, sizeof(double) * n2, hipMemcpyDeviceToDevice, st_));
-    dev_fill(3 * (size_t)na, 0.0, grad.p, st_);
+    dev_fill(9 * (size_t)na, 0.0, grad.p, st_);  // kinetic, nuclear attraction, overlap
     // ---- one-electron part over ordered shell pairs, by (la, lb)
     {
         const auto &sh = basis_.shells;
@@ -686,7 +686,7 @@ void Engine::gradient(double *grad_out, double tau) {
                 dpa.upload(pa.data(), pa.size(), st_);
                 dpb.upload(pb.data(), pb.size(), st_);
                 launch_grad1e(la, lb, d_shells_.p, dpa.p, dpb.p, (int)pa.size(), n, na, d_axyz_.p, d_Z_.p,
-                              d_tab_.p, Dt.p, W.p, grad.p, st_);
+                              d_tab_.p, Dt.p, W.p, grad.p, grad.p + 3 * na, grad.p + 6 * na, st_);
                 PF_HIP(hipStreamSynchronize(st_));
             }
     }
@@ -715,8 +715,9 @@ void Engine::gradient(double *grad_out, double tau) {
         g.tab = d_tab_.p;
         launch_grad2e_class(pairs_[bc].la, pairs_[bc].lb, pairs_[kc].la, pairs_[kc].lb, uhf ? 2 : 1, g, st_);
     }
-    std::vector<double> hg(3 * (size_t)na);
-    grad.download(hg.data(), hg.size(), st_);
+    std::vector<double> h1(9 * (size_t)na), hg(3 * (size_t)na, 0.0), h2(3 * (size_t)na, 0.0),
+        hn(3 * (size_t)na, 0.0);
+    grad.download(h1.data(), h1.size(), st_);
     std::array<std::vector<double>, 3> hp;
     for (int c = 0; c < 3; ++c) {
         hp[c].resize(6 * (size_t)pairs_[c].n);
@@ -727,8 +728,8 @@ void Engine::gradient(double *grad_out, double tau) {
     for (int c = 0; c < 3; ++c)
         for (int i = 0; i < pairs_[c].n; ++i)
             for (int x = 0; x < 3; ++x) {
-                hg[3 * sh[pairs_[c].sa[i]].atom + x] += hp[c][6 * (size_t)i + x];
-                hg[3 * sh[pairs_[c].sb[i]].atom + x] += hp[c][6 * (size_t)i + 3 + x];
+                h2[3 * sh[pairs_[c].sa[i]].atom + x] += hp[c][6 * (size_t)i + x];
+                h2[3 * sh[pairs_[c].sb[i]].atom + x] += hp[c][6 * (size_t)i + 3 + x];
             }
     // nuclear repulsion: d/dR_A sum_{B != A} Z_A Z_B / R_AB
     for (int A = 0; A < na; ++A)
@@ -738,9 +739,16 @@ void Engine::gradient(double *grad_out, double tau) {
             for (int x = 0; x < 3; ++x) { d[x] = mol_.xyz[3 * A + x] - mol_.xyz[3 * B + x]; r2 += d[x] * d[x]; }
             const double r = std::sqrt(r2);
             if (r < 1e-10) continue;
-            for (int x = 0; x < 3; ++x) hg[3 * A + x] -= double(mol_.Z[A]) * mol_.Z[B] * d[x] / (r2 * r);
+            for (int x = 0; x < 3; ++x) hn[3 * A + x] -= double(mol_.Z[A]) * mol_.Z[B] * d[x] / (r2 * r);
         }
+    for (size_t k = 0; k < hg.size(); ++k) hg[k] = hn[k] + h1[k] + h1[3 * na + k] + h1[6 * na + k] + h2[k];
     std::copy(hg.begin(), hg.end(), grad_out);
+    if (terms) {
+        const size_t m = 3 * (size_t)na;
+        std::copy(hn.begin(), hn.end(), terms);
+        std::copy(h1.begin(), h1.end(), terms + m);  // kinetic, nuclear attraction, overlap
+        std::copy(h2.begin(), h2.end(), terms + 4 * m);
+    }
 }
 
 template unsigned long long Engine::build_g<double>(const double *, const double *, double *, double *, bool, int,

@@ -65,6 +65,9 @@ struct Eri {
     static constexpr int NTH = base(GE - 1, GF - 1) + nm(GE - 1, GF - 1);
 
     // Add one primitive quartet to acc[NE*NF] (the contracted [e0|f0]).
+    // COMPACT: tab is a table of orders L .. L+terms-1 only (stride terms),
+    // otherwise the full table (orders 0 .. BOYS_MMAX).
+    template <bool COMPACT = false>
     __device__ __forceinline__ static void prim(R p, R Px, R Py, R Pz, R PAx, R PAy, R PAz, R Kb, R q, R Qx, R Qy,
                                                 R Qz, R QCx, R QCy, R QCz, R Kk, const R *__restrict__ tab,
                                                 R *acc) {
@@ -73,7 +76,8 @@ struct Eri {
         const R PQ[3] = {Px - Qx, Py - Qy, Pz - Qz};
         const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
         R F[L + 1];
-        boys_eval<L, R>(T, tab, F);
+        if constexpr (COMPACT) boys_eval<L, R, BoysTraits<R>::terms, L>(T, tab, F);
+        else boys_eval<L, R>(T, tab, F);
         const R pref = Kb * Kk * sqrt(ipq);
         R th[NTH];
 #pragma unroll

@@ -46,15 +46,28 @@ __device__ __forceinline__ long long quad_index(int i, int j, int k, int l) {
 
 }  // namespace
 
+// Four wavefronts per workgroup, one bra pair each.
+constexpr int JK_BLOCK = 256;
+
 template <int LA, int LB, int LC, int LD, typename R, int NK, int MODE>
-__global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
+__global__ void __launch_bounds__(JK_BLOCK) jk_kernel(JKArgs<R> g) {
     using E = Eri<LA, LB, LC, LD, R>;
     constexpr int NA = E::NA, NB = E::NB, NC = E::NC, ND = E::ND, NINT = E::NINT;
-    const int i = blockIdx.x;
+    constexpr int NT = BoysTraits<R>::terms;
+    // Boys table orders L .. L+NT-1, shared by the workgroup's wavefronts
+    __shared__ R lds_boys[BOYS_NGRID * NT];
+    if constexpr (MODE == MODE_DIRECT || MODE == MODE_STORE || MODE == MODE_PACKED) {
+        for (int k = threadIdx.x; k < BOYS_NGRID * NT; k += JK_BLOCK) {
+            const int gpt = k / NT, m = k - gpt * NT;
+            lds_boys[k] = g.tab[gpt * (BOYS_MMAX + 1) + E::L + m];
+        }
+        __syncthreads();
+    }
+    const int i = blockIdx.x * (JK_BLOCK / 64) + (threadIdx.x >> 6);
     const PairView<R> &B = g.bra;
     const PairView<R> &K = g.ket;
     if (i >= B.n) return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const R Qi = B.Q[i];
 
     int nket;
@@ -122,9 +135,9 @@ __global__ void __launch_bounds__(64) jk_kernel(JKArgs<R> g) {
                         Kk = kq[PF_K * K.npad];
                 for (int v = 0; v < kpb; ++v) {
                     const R *bp = B.prim + (size_t)v * PF_NFIELD * B.npad + i;
-                    E::prim(bp[PF_P * B.npad], bp[PF_PX * B.npad], bp[PF_PY * B.npad], bp[PF_PZ * B.npad],
+                    E::template prim<true>(bp[PF_P * B.npad], bp[PF_PX * B.npad], bp[PF_PY * B.npad], bp[PF_PZ * B.npad],
                             bp[PF_AX * B.npad], bp[PF_AY * B.npad], bp[PF_AZ * B.npad], bp[PF_K * B.npad], q, Qx,
-                            Qy, Qz, QCx, QCy, QCz, Kk, g.tab, acc);
+                            Qy, Qz, QCx, QCy, QCz, Kk, lds_boys, acc);
                 }
             }
             const R CDx = K.AB[j], CDy = K.AB[K.npad + j], CDz = K.AB[2 * K.npad + j];
@@ -288,7 +301,7 @@ __global__ void __launch_bounds__(64) schwarz_kernel(PairView<R> pv, const R *__
 
 template <int LA, int LB, int LC, int LD, typename R>
 void launch_nk_mode(int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
-    const dim3 grid(args.bra.n), block(64);
+    const dim3 grid((args.bra.n + JK_BLOCK / 64 - 1) / (JK_BLOCK / 64)), block(JK_BLOCK);
     if (args.bra.n == 0) return;
     if constexpr (std::is_same<R, double>::value) {
         if (mode == MODE_STORE) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_STORE><<<grid, block, 0, st>>>(args); return; }

@@ -20,7 +20,8 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
                                                      const int *__restrict__ pb, int npairs, int nbf, int natoms,
                                                      const double *__restrict__ axyz, const int *__restrict__ Z,
                                                      const double *__restrict__ tab, const double *__restrict__ D,
-                                                     const double *__restrict__ W, double *grad) {
+                                                     const double *__restrict__ W, double *gT, double *gV,
+                                                     double *gS) {
     constexpr int NA = nc(LA), NB = nc(LB);
     constexpr int LT = LA + LB + 1;
     const int pid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,7 +41,7 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
 #pragma unroll
     for (int d = 0; d < 3; ++d) AB[d] = A.xyz[d] - B.xyz[d];
     const double r2ab = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
-    double gown[3] = {0, 0, 0};
+    double gownT[3] = {0, 0, 0}, gownV[3] = {0, 0, 0}, gownS[3] = {0, 0, 0};
 
     // ---- overlap and kinetic
     for (int i = 0; i < A.np; ++i)
@@ -98,7 +99,8 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
                         }
                         const double dS = pre * (2.0 * a * sp - ac[x] * sm);
                         const double dT = pre * (2.0 * a * tp - ac[x] * tm);
-                        gown[x] += 2.0 * (dab[ia * NB + ib] * dT - wab[ia * NB + ib] * dS);
+                        gownT[x] += 2.0 * dab[ia * NB + ib] * dT;
+                        gownS[x] -= 2.0 * wab[ia * NB + ib] * dS;
                     }
                 }
             }
@@ -158,25 +160,29 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
             }
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            gown[x] += 2.0 * val[x];
+            gownV[x] += 2.0 * val[x];
             const double s = gwave_sum(val[x]);
-            if (threadIdx.x == 0 && s != 0.0) atomicAdd(&grad[3 * c + x], -2.0 * s);
+            if (threadIdx.x == 0 && s != 0.0) atomicAdd(&gV[3 * c + x], -2.0 * s);
         }
     }
     if (active)
 #pragma unroll
-        for (int x = 0; x < 3; ++x) atomicAdd(&grad[3 * A.atom + x], gown[x]);
+        for (int x = 0; x < 3; ++x) {
+            atomicAdd(&gT[3 * A.atom + x], gownT[x]);
+            atomicAdd(&gV[3 * A.atom + x], gownV[x]);
+            atomicAdd(&gS[3 * A.atom + x], gownS[x]);
+        }
 }
 
 void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int *pb, int npairs, int nbf, int natoms,
                    const double *axyz, const int *Z, const double *tab, const double *D, const double *W,
-                   double *grad, hipStream_t st) {
+                   double *gT, double *gV, double *gS, hipStream_t st) {
     if (npairs == 0) return;
     const dim3 grid((npairs + 63) / 64), block(64);
-    if (la == 0 && lb == 0) grad1e_kernel<0, 0><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, grad);
-    else if (la == 0 && lb == 1) grad1e_kernel<0, 1><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, grad);
-    else if (la == 1 && lb == 0) grad1e_kernel<1, 0><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, grad);
-    else grad1e_kernel<1, 1><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, grad);
+    if (la == 0 && lb == 0) grad1e_kernel<0, 0><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
+    else if (la == 0 && lb == 1) grad1e_kernel<0, 1><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
+    else if (la == 1 && lb == 0) grad1e_kernel<1, 0><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
+    else grad1e_kernel<1, 1><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
     PF_HIP(hipGetLastError());
 }
 

@@ -69,7 +69,7 @@ void launch_schwarz(int la, int lb, const PairView<R> &pv, const R *tab, R *Q, h
 struct GradArgs;
 void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int *pb, int npairs, int nbf, int natoms,
                    const double *axyz, const int *Z, const double *tab, const double *D, const double *W,
-                   double *grad, hipStream_t st);
+                   double *gT, double *gV, double *gS, hipStream_t st);
 void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, hipStream_t st);
 
 // ---------------------------------------------------------------- dense FP64

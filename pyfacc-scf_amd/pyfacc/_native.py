@@ -54,7 +54,7 @@ _SIGS = {
     "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
     "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i)]),
     "pf_scf_gradient": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd,
-                             _pd]),
+                             _pd, _pd]),
     "pf_boys": (_i, [_i, _i, _pd, _pd]),
     "pf_eigh": (_i, [_i, _pd, _pd, _pd]),
     "pf_projector": (_i, [_i, _pd, _d, _pd]),

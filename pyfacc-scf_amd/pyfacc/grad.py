@@ -14,16 +14,28 @@ import numpy as np
 from . import _native
 
 
-def scf_gradient(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, **kw):
-    """(energy, gradient (natoms, 3), result dict) from pf_scf_gradient."""
+def scf_gradient(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, want_terms=False, **kw):
+    """(energy, gradient (natoms, 3), result dict) from pf_scf_gradient.
+
+    With ``want_terms`` the result dict also holds ``D`` (converged density,
+    (nd, nbf, nbf)) and ``terms`` (5, natoms, 3): nuclear repulsion, kinetic,
+    nuclear attraction, overlap (-W dS) and two-electron contributions."""
     opt = options if options is not None else _native.default_options(**kw)
     m = _native._Mol(species, coords_bohr)
     g = np.zeros((m.n, 3))
     r = _native.Result()
+    terms = D = None
+    if want_terms:
+        nbf = _native.basis_info(species, coords_bohr, basis, bool(opt.renormalize))["nbf"]
+        D = np.zeros((1 if nalpha == nbeta else 2, nbf, nbf))
+        terms = np.zeros((5, m.n, 3))
     _native._check(_native.load_library().pf_scf_gradient(
         *m.args(), basis.encode(), int(nalpha), int(nbeta), ctypes.byref(opt), ctypes.byref(r),
-        g.ctypes.data_as(_native._pd), None))
-    return r.energy, g, r.as_dict()
+        g.ctypes.data_as(_native._pd), _native._ptr(D), _native._ptr(terms)))
+    out = r.as_dict()
+    if want_terms:
+        out["D"], out["terms"] = D, terms
+    return r.energy, g, out
 
 
 def _function():
