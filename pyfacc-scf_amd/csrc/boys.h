@@ -12,6 +12,7 @@
 // upward recursion F_{m+1} = ((2m+1) F_m - exp(-x)) / (2x), stable for x > m.
 #pragma once
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pf_common.h"
@@ -53,6 +54,91 @@ struct BoysTraits<float> {
     static constexpr int terms = 5;
     __device__ static float ex(float x) { return __expf(-x); }
     __device__ static float rsqrt_pi_over(float x) { return sqrtf(3.14159265358979f / x); }
+};
+
+// 1/sqrt(x) for x > 0 in the normal range: hardware estimate, then (FP64) two
+// Newton steps y <- y (3/2 - x y^2 / 2), each squaring the relative error.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-h, y * y, 0.5), y);
+    y = fma(y, fma(-h, y * y, 0.5), y);
+    return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Workgroup copy of the Boys table for one top order M (ERI kernels).
+// Grid point k holds the Taylor coefficients of F_M about x_k, pre-divided
+// by j!, c_j = F_{M+j}(x_k) / j! (j < terms), and, for M > 0, exp(-x_k), so
+// that the evaluation needs neither factorials nor an exp call:
+//   F_M(x)    = sum_j c_j d^j,  d = x_k - x, |d| <= h/2
+//   exp(-x)   = exp(-x_k) exp(d), exp(d) by its Taylor series to d^8 in
+//               FP64 ((1/32)^9 / 9! < 1e-19), to d^5 in FP32
+template <int M, typename R>
+struct BoysLds {
+    static constexpr int NT = BoysTraits<R>::terms;
+    static constexpr int STRIDE = NT + (M > 0 ? 1 : 0);
+    static constexpr int SIZE = BOYS_NGRID * STRIDE;
+
+    // Fill lds[SIZE] from the full table (stride BOYS_MMAX+1); all threads of
+    // the workgroup take part, the caller synchronises.
+    __device__ static void fill(R *lds, const R *__restrict__ tab, int tid, int nthreads) {
+        for (int k = tid; k < SIZE; k += nthreads) {
+            const int g = k / STRIDE, j = k - g * STRIDE;
+            R v;
+            if (j < NT) {
+                R f = R(1);
+                for (int t = 2; t <= j; ++t) f *= R(t);
+                v = tab[g * (BOYS_MMAX + 1) + M + j] / f;
+            } else {
+                v = R(exp(-double(g) * BOYS_H));
+            }
+            lds[k] = v;
+        }
+    }
+
+    __device__ __forceinline__ static void eval(R x, const R *lds, R *F) {
+        if (x < R(BOYS_XMAX)) {
+            const int k = int(x * R(1.0 / BOYS_H) + R(0.5));
+            const R d = R(k) * R(BOYS_H) - x;
+            const R *t = lds + k * STRIDE;
+            R acc = t[NT - 1];
+#pragma unroll
+            for (int j = NT - 2; j >= 0; --j) acc = fma(acc, d, t[j]);
+            F[M] = acc;
+            if constexpr (M > 0) {
+                R ed;
+                if constexpr (std::is_same<R, double>::value) {
+                    ed = fma(R(1.0 / 40320.0), d, R(1.0 / 5040.0));
+                    ed = fma(ed, d, R(1.0 / 720.0));
+                    ed = fma(ed, d, R(1.0 / 120.0));
+                } else {
+                    ed = R(1.0 / 120.0);
+                }
+                ed = fma(ed, d, R(1.0 / 24.0));
+                ed = fma(ed, d, R(1.0 / 6.0));
+                ed = fma(ed, d, R(0.5));
+                ed = fma(ed, d, R(1));
+                ed = fma(ed, d, R(1));
+                const R e = t[NT] * ed;
+                const R x2 = x + x;
+#pragma unroll
+                for (int m = M - 1; m >= 0; --m) F[m] = fma(x2, F[m + 1], e) * R(1.0 / (2 * m + 1));
+            }
+        } else {
+            // F_0 = sqrt(pi/x)/2; upward F_{m+1} = (2m+1)/(2x) F_m.  The
+            // exp(-x) term of the exact recursion is below exp(-40)/80 < 6e-20
+            // in absolute value, far under the integral accuracy.
+            static_assert(M <= 6, "asymptotic branch without exp(-x) is used up to order 6");
+            const R r = rsqrt_nr(x);
+            F[0] = R(0.88622692545275801365) * r;  // sqrt(pi)/2
+            if constexpr (M > 0) {
+                const R o2x = R(0.5) * r * r;
+#pragma unroll
+                for (int m = 0; m < M; ++m) F[m + 1] = F[m] * (R(2 * m + 1) * o2x);
+            }
+        }
+    }
 };
 
 // Device: F[0..M] at x.  tab points to the table in R precision: row k holds

@@ -20,7 +20,7 @@ struct PairSet {
     int n = 0, npad = 0, kp = 0;
     std::vector<int> sa, sb;   // host copies, sorted by Q descending
     std::vector<double> Q;
-    DevBuf<int> d_sa, d_sb, d_oa, d_ob;
+    DevBuf<int> d_sa, d_sb, d_oa, d_ob, d_kpn;
     DevBuf<double> d_Q, d_AB, d_prim;
     DevBuf<float> f_Q, f_AB, f_prim;
     PairView<double> view64() const;
@@ -104,6 +104,7 @@ class Engine {
     DevBuf<float> Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;
     DevBuf<double> Dj64_;
     DevBuf<unsigned long long> count_;
+    std::array<DevBuf<int>, 6> plan_nket_, plan_off_, plan_work_;  // direct-mode work plans per class
     // stored integrals
     bool store_ready_ = false;
     std::array<DevBuf<double>, 6> store_;

@@ -39,6 +39,7 @@ PairView<double> PairSet::view64() const {
     v.n = n; v.npad = npad; v.kp = kp;
     v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
     v.Q = d_Q.p; v.AB = d_AB.p; v.prim = d_prim.p;
+    v.kpn = d_kpn.p;
     return v;
 }
 PairView<float> PairSet::view32() const {
@@ -46,6 +47,7 @@ PairView<float> PairSet::view32() const {
     v.n = n; v.npad = npad; v.kp = kp;
     v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
     v.Q = f_Q.p; v.AB = f_AB.p; v.prim = f_prim.p;
+    v.kpn = d_kpn.p;
     return v;
 }
 
@@ -109,35 +111,60 @@ void Engine::setup_pairs() {
             if (sh[b].l > sh[a].l) std::swap(f, s);
             all[sh[f].l + sh[s].l].push_back({f, s});
         }
-    auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list) {
+    // Primitive pairs of a shell pair are stored by bound |K|/sqrt(p)
+    // descending; kpn counts those with bound * (largest bound of any pair) >=
+    // PRIM_EPS, the only ones the ERI kernels visit: a primitive quartet is
+    // at most K_ab K_cd / sqrt(p+q) times the polynomial factors of the p-type
+    // recurrences, so the dropped ones are below 1e-16 in magnitude up to
+    // those factors.  kref < 0: keep every primitive (Schwarz pass).
+    constexpr double PRIM_EPS = 1e-16;
+    double bound_max = 0.0;
+    auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list, double kref) {
         ps.n = (int)list.size();
         ps.npad = round64(std::max(ps.n, 1));
         int kp = 1;
         for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
         ps.kp = kp;
-        std::vector<int> sa(ps.n), sb(ps.n), oa(ps.n), ob(ps.n);
+        std::vector<int> sa(ps.n), sb(ps.n), oa(ps.n), ob(ps.n), kpn(ps.npad, 0);
         std::vector<double> AB(3 * (size_t)ps.npad, 0.0), prim((size_t)kp * PF_NFIELD * ps.npad, 0.0);
         for (int u = 0; u < kp; ++u)  // padding entries: unit exponent, zero weight
-            for (int i = 0; i < ps.npad; ++i) prim[((size_t)u * PF_NFIELD + PF_P) * ps.npad + i] = 1.0;
+            for (int i = 0; i < ps.npad; ++i) {
+                prim[((size_t)u * PF_NFIELD + PF_P) * ps.npad + i] = 1.0;
+                prim[((size_t)u * PF_NFIELD + PF_O2P) * ps.npad + i] = 0.5;
+            }
         const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
         for (int i = 0; i < ps.n; ++i) {
             const Shell &A = sh[list[i].first], &B = sh[list[i].second];
             sa[i] = list[i].first; sb[i] = list[i].second; oa[i] = A.bf; ob[i] = B.bf;
             double ab[3], r2 = 0.0;
             for (int d = 0; d < 3; ++d) { ab[d] = A.xyz[d] - B.xyz[d]; r2 += ab[d] * ab[d]; AB[d * ps.npad + i] = ab[d]; }
-            int u = 0;
+            std::vector<std::array<double, 3>> order;  // (bound, x, y)
             for (int x = 0; x < A.np; ++x)
-                for (int y = 0; y < B.np; ++y, ++u) {
+                for (int y = 0; y < B.np; ++y) {
                     const double a = A.exp[x], b = B.exp[y], p = a + b;
-                    double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
-                    col[PF_P * ps.npad] = p;
-                    for (int d = 0; d < 3; ++d) {
-                        const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
-                        col[(PF_PX + d) * ps.npad] = P;
-                        col[(PF_AX + d) * ps.npad] = P - A.xyz[d];
-                    }
-                    col[PF_K * ps.npad] = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+                    const double K = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+                    order.push_back({std::fabs(K) / std::sqrt(p), (double)x, (double)y});
                 }
+            std::stable_sort(order.begin(), order.end(), [](const auto &l, const auto &r) { return l[0] > r[0]; });
+            for (const auto &o : order) {
+                bound_max = std::max(bound_max, o[0]);
+                if (kref < 0.0 || o[0] * kref >= PRIM_EPS) ++kpn[i];
+            }
+            int u = 0;
+            for (const auto &o : order) {
+                const int x = (int)o[1], y = (int)o[2];
+                const double a = A.exp[x], b = B.exp[y], p = a + b;
+                double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
+                col[PF_P * ps.npad] = p;
+                for (int d = 0; d < 3; ++d) {
+                    const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
+                    col[(PF_PX + d) * ps.npad] = P;
+                    col[(PF_AX + d) * ps.npad] = P - A.xyz[d];
+                }
+                col[PF_K * ps.npad] = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+                col[PF_O2P * ps.npad] = 0.5 / p;
+                ++u;
+            }
             for (; u < kp; ++u) {  // padding primitives contribute zero
                 double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
                 col[PF_P * ps.npad] = 1.0;
@@ -147,6 +174,7 @@ void Engine::setup_pairs() {
         ps.sa = sa; ps.sb = sb;
         ps.d_sa.upload(sa.data(), ps.n, st_); ps.d_sb.upload(sb.data(), ps.n, st_);
         ps.d_oa.upload(oa.data(), ps.n, st_); ps.d_ob.upload(ob.data(), ps.n, st_);
+        ps.d_kpn.upload(kpn.data(), kpn.size(), st_);
         ps.d_AB.upload(AB.data(), AB.size(), st_);
         ps.d_prim.upload(prim.data(), prim.size(), st_);
         std::vector<float> fAB(AB.begin(), AB.end()), fprim(prim.begin(), prim.end());
@@ -159,7 +187,7 @@ void Engine::setup_pairs() {
     for (int c = 0; c < 3; ++c) {
         PairSet &ps = pairs_[c];
         ps.la = cls_l[c][0]; ps.lb = cls_l[c][1];
-        fill(ps, all[c]);
+        fill(ps, all[c], -1.0);
         ps.d_Q.alloc(std::max(ps.n, 1));
         launch_schwarz<double>(ps.la, ps.lb, ps.view64(), d_tab_.p, ps.d_Q.p, st_);
         Qall[c].resize(ps.n);
@@ -178,7 +206,7 @@ void Engine::setup_pairs() {
         std::vector<double> q;
         for (int i : idx) { kept.push_back(all[c][i]); q.push_back(Qall[c][i]); }
         PairSet &ps = pairs_[c];
-        fill(ps, kept);
+        fill(ps, kept, bound_max);
         ps.Q = q;
         ps.d_Q.upload(q.data(), q.size(), st_);
         std::vector<float> fq(q.begin(), q.end());
@@ -186,15 +214,14 @@ void Engine::setup_pairs() {
         for (auto &v : fq) v = std::nextafter(v, 1e30f);
         ps.f_Q.upload(fq.data(), fq.size(), st_);
     }
-    // One launch per unordered class; for mixed classes the bra is the pair
-    // class with more pairs, so the grid (one wavefront per bra pair) is wide
-    // and the longer ket loop runs over the shorter list.
+    // One launch per unordered class.
     classes_.clear();
     int slot = 0;
     for (int x = 0; x < 3; ++x)
         for (int y = 0; y <= x; ++y, ++slot) {
+            // bra = the pair class of higher angular momentum: its exchange
+            // rows are summed on chip, only J'_cd of the ket goes to memory
             QuartetClass qc{x, y, slot};
-            if (x != y && pairs_[y].n > pairs_[x].n) std::swap(qc.bra, qc.ket);
             classes_.push_back(qc);
             const int l4[4] = {pairs_[qc.bra].la, pairs_[qc.bra].lb, pairs_[qc.ket].la, pairs_[qc.ket].lb};
             for (int k = 0; k < 4; ++k) prof_class_l[slot * 4 + k] = l4[k];
@@ -203,6 +230,10 @@ void Engine::setup_pairs() {
     J64_.alloc(n2); K64a_.alloc(n2); K64b_.alloc(n2); Dj64_.alloc(n2);
     Dsh64_.alloc((size_t)nshell() * nshell()); dmax64_.alloc(1);
     count_.alloc(6);
+    for (size_t q = 0; q < classes_.size(); ++q) {  // direct-mode work plans
+        const int nb = std::max(pairs_[classes_[q].bra].n, 1);
+        plan_nket_[q].alloc(nb); plan_off_[q].alloc(nb + 1); plan_work_[q].alloc(2);
+    }
     PF_HIP(hipStreamSynchronize(st_));
     pairs_ready_ = true;
 }
@@ -306,6 +337,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
             }
         }
         a.count = count_.p + classes_[q].slot;
+        a.nket = plan_nket_[q].p; a.item_off = plan_off_[q].p; a.work = plan_work_[q].p;
         if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
@@ -620,6 +652,8 @@ void Engine::setup_grad_pairs() {
             for (int i = 0; i < ps.npad; ++i) {
                 const size_t o = ((size_t)u * PF_NFIELD + PF_P) * ps.npad + i;
                 w[o] = rw[o] = r[o] = 1.0;
+                const size_t o2 = ((size_t)u * PF_NFIELD + PF_O2P) * ps.npad + i;
+                w[o2] = rw[o2] = r[o2] = 0.5;
             }
         for (int i = 0; i < ps.n; ++i) {
             const Shell &A = sh[ps.sa[i]], &B = sh[ps.sb[i]];
@@ -639,6 +673,7 @@ void Engine::setup_grad_pairs() {
                             col[(PF_AX + d) * ps.npad] = P - (arr == &w ? A.xyz[d] : B.xyz[d]);
                         }
                         col[PF_K * ps.npad] = arr == &w ? 2.0 * a * K : (arr == &rw ? 2.0 * b * K : K);
+                        col[PF_O2P * ps.npad] = 0.5 / p;
                     }
                 }
             for (; u < ps.kp; ++u)

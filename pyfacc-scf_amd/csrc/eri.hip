@@ -23,6 +23,33 @@ extern template void launch_nk_mode<1, 0, 1, 1, float>(int, int, const JKArgs<fl
 extern template void launch_nk_mode<1, 1, 1, 1, double>(int, int, const JKArgs<double> &, hipStream_t);
 extern template void launch_nk_mode<1, 1, 1, 1, float>(int, int, const JKArgs<float> &, hipStream_t);
 
+__global__ void __launch_bounds__(1024) jk_scan_kernel(int n, int *item_off, int *work) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x, per = (n + 1023) / 1024;
+    const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; ++i) s += item_off[i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the per-thread sums
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int off = part[t] - s;
+    for (int i = b0; i < b1; ++i) {
+        const int c = item_off[i];
+        item_off[i] = off;
+        off += c;
+    }
+    if (t == 1023) {
+        item_off[n] = part[1023];
+        work[0] = 0;
+        work[1] = part[1023];
+    }
+}
+
 template <typename R>
 void launch_jk_class(int la, int lb, int lc, int ld, int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
     const int key = ((la * 2 + lb) * 2 + lc) * 2 + ld;

@@ -65,27 +65,28 @@ struct Eri {
     static constexpr int NTH = base(GE - 1, GF - 1) + nm(GE - 1, GF - 1);
 
     // Add one primitive quartet to acc[NE*NF] (the contracted [e0|f0]).
-    // COMPACT: tab is a table of orders L .. L+terms-1 only (stride terms),
-    // otherwise the full table (orders 0 .. BOYS_MMAX).
-    template <bool COMPACT = false>
-    __device__ __forceinline__ static void prim(R p, R Px, R Py, R Pz, R PAx, R PAy, R PAz, R Kb, R q, R Qx, R Qy,
-                                                R Qz, R QCx, R QCy, R QCz, R Kk, const R *__restrict__ tab,
-                                                R *acc) {
-        const R pq = p + q, ipq = R(1) / pq;
-        const R rho = p * q * ipq;
+    // o2p = 1/(2p), o2q = 1/(2q).  LDS: tab is a workgroup table laid out by
+    // BoysLds<L, R>, otherwise the full table (orders 0 .. BOYS_MMAX).
+    template <bool LDS = false>
+    __device__ __forceinline__ static void prim(R p, R o2p, R Px, R Py, R Pz, R PAx, R PAy, R PAz, R Kb, R q, R o2q,
+                                                R Qx, R Qy, R Qz, R QCx, R QCy, R QCz, R Kk,
+                                                const R *__restrict__ tab, R *acc) {
+        // r = 1/sqrt(p+q) gives both 1/(p+q) and the prefactor's square root
+        const R r = rsqrt_nr(p + q), ipq = r * r;
+        const R rop = q * ipq, roq = p * ipq;  // rho/p, rho/q
+        const R rho = p * rop;
         const R PQ[3] = {Px - Qx, Py - Qy, Pz - Qz};
         const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
         R F[L + 1];
-        if constexpr (COMPACT) boys_eval<L, R, BoysTraits<R>::terms, L>(T, tab, F);
+        if constexpr (LDS) BoysLds<L, R>::eval(T, tab, F);
         else boys_eval<L, R>(T, tab, F);
-        const R pref = Kb * Kk * sqrt(ipq);
+        const R pref = Kb * Kk * r;
         R th[NTH];
 #pragma unroll
         for (int m = 0; m <= L; ++m) th[m] = pref * F[m];
         if constexpr (LAB > 0) {
             const R PA[3] = {PAx, PAy, PAz};
-            const R WP[3] = {-q * ipq * PQ[0], -q * ipq * PQ[1], -q * ipq * PQ[2]};
-            const R o2p = R(0.5) / p, rop = rho / p;
+            const R WP[3] = {-rop * PQ[0], -rop * PQ[1], -rop * PQ[2]};
 #pragma unroll
             for (int g = 1; g < GE; ++g) {
                 const int d = cdir(g), g1 = cminus(g, d), em = ccomp(g, d) - 1;
@@ -103,8 +104,8 @@ struct Eri {
         }
         if constexpr (LCD > 0) {
             const R QC[3] = {QCx, QCy, QCz};
-            const R WQ[3] = {p * ipq * PQ[0], p * ipq * PQ[1], p * ipq * PQ[2]};
-            const R o2q = R(0.5) / q, roq = rho / q, o2pq = R(0.5) * ipq;
+            const R WQ[3] = {roq * PQ[0], roq * PQ[1], roq * PQ[2]};
+            const R o2pq = R(0.5) * ipq;
 #pragma unroll
             for (int h = 1; h < GF; ++h) {
                 const int d = cdir(h), h1 = cminus(h, d), fm = ccomp(h, d) - 1;

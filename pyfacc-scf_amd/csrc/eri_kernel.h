@@ -1,24 +1,30 @@
 // Shell-quartet ERI kernels with the fused J/K digest.
 //
-// Work decomposition (one launch per quartet class (LA LB|LC LD), bra class
-// >= ket class): one 64-lane wavefront per bra shell pair, lanes stride over
-// that pair's ket list.  Pairs of each class are sorted by Schwarz bound Q
-// descending, so the kets that survive Q_i Q_j dmax >= tau form a prefix found
-// by binary search; a per-lane density-weighted test then removes the rest.
-// The bra pair's primitive data is wavefront-uniform (scalar loads); ket data
-// is structure-of-arrays so the 64 lanes read consecutive addresses.
+// One launch per quartet class (LA LB|LC LD); the bra pair class is the one
+// of higher angular momentum.  Pairs of each class are sorted by Schwarz
+// bound Q descending, so the kets that can pass Q_i Q_j dmax >= tau form a
+// prefix of the ket list.  The bra pair's data is uniform across a
+// wavefront (scalar loads); ket data is structure-of-arrays so the 64 lanes
+// (one ket each) read consecutive addresses.
 //
 // J/K digest for a unique shell quartet with degeneracy factor
 // deg = (1/2)^{[a=b] + [c=d] + [ab=cd]}, into non-symmetric accumulators
 // (symmetrised afterwards: J = J' + J'^T, K = K' + K'^T):
-//   J'_ab += 2 deg (ab|cd) D_cd      (wavefront-reduced, one atomic per element)
-//   J'_cd += 2 deg (ab|cd) D_ab
+//   J'_ab += 2 deg (ab|cd) D_cd      (reduced over the kets of the bra)
+//   J'_cd += 2 deg (ab|cd) D_ab      (device atomics)
 //   K'_ac += deg (ab|cd) D_bd,  K'_ad += deg (ab|cd) D_bc,
 //   K'_bc += deg (ab|cd) D_ad,  K'_bd += deg (ab|cd) D_ac
+// Every exchange term lands on a row of the bra (a or b).  Direct builds
+// (jk_direct_kernel) give each workgroup one bra pair and a block of its kets
+// at a time and sum those rows, and J'_ab, in LDS, where FP64 atomics run at
+// LDS rate (scattered FP64 atomics to device memory sustain only ~24e9/s);
+// the rows go to device memory once per block.
 // This replaces the reference's G loop (rscf.f90:121-134, uscf.f90:149-175),
 // which gathers nbf^4 values from a stored list and drops every
 // non-canonical permutation.
 #pragma once
+#include <algorithm>
+
 #include "eri_core.h"
 #include "kernels.h"
 
@@ -46,54 +52,179 @@ __device__ __forceinline__ long long quad_index(int i, int j, int k, int l) {
 
 }  // namespace
 
-// Four wavefronts per workgroup, one bra pair each.
-constexpr int JK_BLOCK = 256;
+// Wavefronts per workgroup: eight (two per SIMD, sharing one LDS Boys table)
+// where the class fits in 256 registers, four for the register-heavy classes
+// of total angular momentum >= 3.
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_waves() { return LA + LB + LC + LD >= 3 ? 4 : 8; }
+// Kets per direct-mode work item (one workgroup): a few passes of every
+// wavefront, so that the per-item cost (queue atomic, row flush) is small
+// beside the item's integrals while items stay short enough to balance.
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_chunk() {
+    constexpr int L = LA + LB + LC + LD;
+    return 64 * jk_waves<LA, LB, LC, LD>() * (L <= 1 ? 8 : (L <= 3 ? 4 : 2));
+}
 
-template <int LA, int LB, int LC, int LD, typename R, int NK, int MODE>
-__global__ void __launch_bounds__(JK_BLOCK) jk_kernel(JKArgs<R> g) {
+// Contracted (ab|cd) block, eri[((ia*NB+ib)*NC+ic)*ND+id], of bra pair i
+// (uniform; its primitive data through bprim) and ket pair j (this lane).
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void eri_block(const PairView<R> &B, const R *__restrict__ bprim, int i, R ABx, R ABy,
+                                          R ABz, const PairView<R> &K, int j, const R *lds_boys, R *eri) {
     using E = Eri<LA, LB, LC, LD, R>;
-    constexpr int NA = E::NA, NB = E::NB, NC = E::NC, ND = E::ND, NINT = E::NINT;
-    constexpr int NT = BoysTraits<R>::terms;
-    // Boys table orders L .. L+NT-1, shared by the workgroup's wavefronts
-    __shared__ R lds_boys[BOYS_NGRID * NT];
-    if constexpr (MODE == MODE_DIRECT || MODE == MODE_STORE || MODE == MODE_PACKED) {
-        for (int k = threadIdx.x; k < BOYS_NGRID * NT; k += JK_BLOCK) {
-            const int gpt = k / NT, m = k - gpt * NT;
-            lds_boys[k] = g.tab[gpt * (BOYS_MMAX + 1) + E::L + m];
+    R acc[E::NACC];
+#pragma unroll
+    for (int k = 0; k < E::NACC; ++k) acc[k] = R(0);
+    // significant primitive pairs only (bra: uniform count; ket: this lane's)
+    const int kpk = K.kpn[j], kpb = __builtin_amdgcn_readfirstlane(B.kpn[i]);
+    for (int u = 0; u < kpk; ++u) {
+        const R *kq = K.prim + (size_t)u * PF_NFIELD * K.npad + j;
+        const R q = kq[PF_P * K.npad], Qx = kq[PF_PX * K.npad], Qy = kq[PF_PY * K.npad], Qz = kq[PF_PZ * K.npad];
+        const R QCx = kq[PF_AX * K.npad], QCy = kq[PF_AY * K.npad], QCz = kq[PF_AZ * K.npad],
+                Kk = kq[PF_K * K.npad], o2q = kq[PF_O2P * K.npad];
+        for (int v = 0; v < kpb; ++v) {
+            // bra data: read-only in these kernels and wavefront-uniform -> scalar loads
+            const R *bp = bprim + (size_t)v * PF_NFIELD * B.npad + i;
+            E::template prim<true>(bp[PF_P * B.npad], bp[PF_O2P * B.npad], bp[PF_PX * B.npad], bp[PF_PY * B.npad],
+                                   bp[PF_PZ * B.npad], bp[PF_AX * B.npad], bp[PF_AY * B.npad], bp[PF_AZ * B.npad],
+                                   bp[PF_K * B.npad], q, o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, lds_boys, acc);
         }
+    }
+    const R CDx = K.AB[j], CDy = K.AB[K.npad + j], CDz = K.AB[2 * K.npad + j];
+    E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, eri);
+}
+
+// J/K digest of one quartet (bra functions oa.., ob.., ket functions oc..,
+// od..).  jab[NA*NB] accumulates J'_ab (reduced by the caller); J'_cd goes
+// to device memory; kadd(kk, r, col, v) adds v to K'(row r, col) of density
+// kk, bra row r < NA being function oa + r and r >= NA function ob + r - NA.
+template <int LA, int LB, int LC, int LD, typename R, int NK, typename KAdd>
+__device__ __forceinline__ void digest(const JKArgs<R> &g, const R *eri, int oa, int ob, int oc, int od, R deg,
+                                       const R *dab, R *jab, KAdd &&kadd) {
+    using E = Eri<LA, LB, LC, LD, R>;
+    constexpr int NA = E::NA, NB = E::NB, NC = E::NC, ND = E::ND;
+    const int nbf = g.nbf;
+    const R deg2 = deg + deg;
+    // ---- Coulomb
+    R dcd[NC * ND];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = g.Dj[(size_t)(oc + ic) * nbf + od + id];
+#pragma unroll
+    for (int ab = 0; ab < NA * NB; ++ab) {
+        R s = R(0);
+#pragma unroll
+        for (int cd = 0; cd < NC * ND; ++cd) s += eri[ab * NC * ND + cd] * dcd[cd];
+        jab[ab] += deg2 * s;
+    }
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) {
+            R s = R(0);
+#pragma unroll
+            for (int ab = 0; ab < NA * NB; ++ab) s += eri[ab * NC * ND + ic * ND + id] * dab[ab];
+            atomicAdd(&g.J[(size_t)(oc + ic) * nbf + od + id], double(deg2 * s));
+        }
+    // ---- exchange, one pass per density
+#pragma unroll
+    for (int kk = 0; kk < NK; ++kk) {
+        const R *Dk = kk == 0 ? g.Dk0 : g.Dk1;
+        R dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = Dk[(size_t)(ob + ib) * nbf + od + id];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = Dk[(size_t)(ob + ib) * nbf + oc + ic];
+        }
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+            for (int id = 0; id < ND; ++id) dad[ia * ND + id] = Dk[(size_t)(oa + ia) * nbf + od + id];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) dac[ia * NC + ic] = Dk[(size_t)(oa + ia) * nbf + oc + ic];
+        }
+#define ERI_(ia, ib, ic, id) eri[(((ia) * NB + (ib)) * NC + (ic)) * ND + (id)]
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+                R s = R(0);
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) s += ERI_(ia, ib, ic, id) * dbd[ib * ND + id];
+                kadd(kk, ia, oc + ic, deg * s);  // K'_ac
+            }
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                R s = R(0);
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) s += ERI_(ia, ib, ic, id) * dbc[ib * NC + ic];
+                kadd(kk, ia, od + id, deg * s);  // K'_ad
+            }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+                R s = R(0);
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) s += ERI_(ia, ib, ic, id) * dad[ia * ND + id];
+                kadd(kk, NA + ib, oc + ic, deg * s);  // K'_bc
+            }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                R s = R(0);
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) s += ERI_(ia, ib, ic, id) * dac[ia * NC + ic];
+                kadd(kk, NA + ib, od + id, deg * s);  // K'_bd
+            }
+#undef ERI_
+    }
+}
+
+// Bra-major kernel for the stored-integral and export modes: one wavefront
+// per bra pair, lanes over its kets (store layout: per bra, blocks of 64
+// kets, integral k of lane l at k*64 + l).
+//   MODE_STORE:  compute and store the quartets that pass Schwarz (FP64)
+//   MODE_LOAD:   read them back and digest into J', K' (device atomics)
+//   MODE_PACKED: compute every unique quartet into the canonical packed list
+template <int LA, int LB, int LC, int LD, typename R, int NK, int MODE>
+__global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(JKArgs<R> g, const R *__restrict__ bprim) {
+    constexpr int W = jk_waves<LA, LB, LC, LD>();
+    using E = Eri<LA, LB, LC, LD, R>;
+    using BT = BoysLds<E::L, R>;
+    constexpr int NA = E::NA, NB = E::NB, NC = E::NC, ND = E::ND, NINT = E::NINT;
+    constexpr bool COMPUTE = MODE == MODE_STORE || MODE == MODE_PACKED;
+    __shared__ R lds_boys[COMPUTE ? BT::SIZE : 1];
+    if constexpr (COMPUTE) {
+        BT::fill(lds_boys, g.tab, threadIdx.x, 64 * W);
         __syncthreads();
     }
-    const int i = blockIdx.x * (JK_BLOCK / 64) + (threadIdx.x >> 6);
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * W + (threadIdx.x >> 6));
     const PairView<R> &B = g.bra;
     const PairView<R> &K = g.ket;
     if (i >= B.n) return;
     const int lane = threadIdx.x & 63;
-    const R Qi = B.Q[i];
-
-    int nket;
-    if constexpr (MODE == MODE_LOAD || MODE == MODE_STORE) {
-        nket = g.store_nket[i];
-    } else if constexpr (MODE == MODE_PACKED) {
-        nket = g.same ? i + 1 : K.n;
-    } else {
-        const R qthr = g.tau / (Qi * (g.dmax ? *g.dmax : R(1)));
-        int lo = 0, hi = K.n;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (K.Q[mid] >= qthr) lo = mid + 1;
-            else hi = mid;
-        }
-        nket = g.same ? min(lo, i + 1) : lo;
-    }
-
+    const int nbf = g.nbf;
+    const int nket = MODE == MODE_PACKED ? (g.same ? i + 1 : K.n) : g.store_nket[i];
     const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
     const R ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
     const R deg_ab = (a == b) ? R(0.5) : R(1);
-    const int nbf = g.nbf;
-
-    R dab[NA * NB];
-    R jab[NA * NB];
-    if constexpr (MODE == MODE_DIRECT || MODE == MODE_LOAD) {
+    R dab[NA * NB], jab[NA * NB];
+    if constexpr (MODE == MODE_LOAD) {
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -103,53 +234,24 @@ __global__ void __launch_bounds__(JK_BLOCK) jk_kernel(JKArgs<R> g) {
             }
     }
     unsigned long long counted = 0;
-
     for (int j0 = 0; j0 < nket; j0 += 64) {
         const int j = j0 + lane;
         if (j >= nket) break;
-        const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
-        if constexpr (MODE == MODE_DIRECT) {
-            if (g.Dsh) {
-                const int ns = g.nsh;
-                R dq = R(2) * max(g.Dsh[a * ns + b], g.Dsh[c * ns + d]);
-                dq = max(dq, max(max(g.Dsh[b * ns + d], g.Dsh[b * ns + c]), max(g.Dsh[a * ns + d], g.Dsh[a * ns + c])));
-                if (Qi * K.Q[j] * dq < g.tau) continue;
-            }
-        }
         ++counted;
         R eri[NINT];
         if constexpr (MODE == MODE_LOAD) {
-            const R *src = g.store + g.store_off[i] + (size_t)(j0 >> 6) * 64 * NINT + lane;
+            const R *src = g.store + g.store_off[i] + (size_t)j0 * NINT + lane;
 #pragma unroll
             for (int k = 0; k < NINT; ++k) eri[k] = src[k * 64];
         } else {
-            R acc[E::NACC];
-#pragma unroll
-            for (int k = 0; k < E::NACC; ++k) acc[k] = R(0);
-            const int kpk = K.kp, kpb = B.kp;
-            for (int u = 0; u < kpk; ++u) {
-                const R *kq = K.prim + (size_t)u * PF_NFIELD * K.npad + j;
-                const R q = kq[PF_P * K.npad], Qx = kq[PF_PX * K.npad], Qy = kq[PF_PY * K.npad],
-                        Qz = kq[PF_PZ * K.npad];
-                const R QCx = kq[PF_AX * K.npad], QCy = kq[PF_AY * K.npad], QCz = kq[PF_AZ * K.npad],
-                        Kk = kq[PF_K * K.npad];
-                for (int v = 0; v < kpb; ++v) {
-                    const R *bp = B.prim + (size_t)v * PF_NFIELD * B.npad + i;
-                    E::template prim<true>(bp[PF_P * B.npad], bp[PF_PX * B.npad], bp[PF_PY * B.npad], bp[PF_PZ * B.npad],
-                            bp[PF_AX * B.npad], bp[PF_AY * B.npad], bp[PF_AZ * B.npad], bp[PF_K * B.npad], q, Qx,
-                            Qy, Qz, QCx, QCy, QCz, Kk, lds_boys, acc);
-                }
-            }
-            const R CDx = K.AB[j], CDy = K.AB[K.npad + j], CDz = K.AB[2 * K.npad + j];
-            E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, eri);
+            eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
         }
         if constexpr (MODE == MODE_STORE) {
-            R *dst = g.store + g.store_off[i] + (size_t)(j0 >> 6) * 64 * NINT + lane;
+            R *dst = g.store + g.store_off[i] + (size_t)j0 * NINT + lane;
 #pragma unroll
             for (int k = 0; k < NINT; ++k) dst[k * 64] = eri[k];
-            continue;
-        }
-        if constexpr (MODE == MODE_PACKED) {
+        } else if constexpr (MODE == MODE_PACKED) {
+            const int oc = K.oa[j], od = K.ob[j];
 #pragma unroll
             for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -160,108 +262,170 @@ __global__ void __launch_bounds__(JK_BLOCK) jk_kernel(JKArgs<R> g) {
                         for (int id = 0; id < ND; ++id)
                             g.packed[quad_index(oa + ia, ob + ib, oc + ic, od + id)] =
                                 double(eri[((ia * NB + ib) * NC + ic) * ND + id]);
-            continue;
-        }
-        if constexpr (MODE == MODE_DIRECT || MODE == MODE_LOAD) {
+        } else {
+            const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
-            const R deg2 = deg + deg;
-            // ---- Coulomb
-            R dcd[NC * ND];
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = g.Dj[(size_t)(oc + ic) * nbf + od + id];
-#pragma unroll
-            for (int ab = 0; ab < NA * NB; ++ab) {
-                R s = R(0);
-#pragma unroll
-                for (int cd = 0; cd < NC * ND; ++cd) s += eri[ab * NC * ND + cd] * dcd[cd];
-                jab[ab] += deg2 * s;
-            }
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-                for (int id = 0; id < ND; ++id) {
-                    R s = R(0);
-#pragma unroll
-                    for (int ab = 0; ab < NA * NB; ++ab) s += eri[ab * NC * ND + ic * ND + id] * dab[ab];
-                    atomicAdd(&g.J[(size_t)(oc + ic) * nbf + od + id], double(deg2 * s));
-                }
-            // ---- exchange, one pass per density
-#pragma unroll
-            for (int kk = 0; kk < NK; ++kk) {
-                const R *Dk = kk == 0 ? g.Dk0 : g.Dk1;
-                double *Kout = kk == 0 ? g.K0 : g.K1;
-                R dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib) {
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = Dk[(size_t)(ob + ib) * nbf + od + id];
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = Dk[(size_t)(ob + ib) * nbf + oc + ic];
-                }
-#pragma unroll
-                for (int ia = 0; ia < NA; ++ia) {
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) dad[ia * ND + id] = Dk[(size_t)(oa + ia) * nbf + od + id];
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) dac[ia * NC + ic] = Dk[(size_t)(oa + ia) * nbf + oc + ic];
-                }
-#define ERI_(ia, ib, ic, id) eri[(((ia) * NB + (ib)) * NC + (ic)) * ND + (id)]
-#pragma unroll
-                for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) {
-                        R s = R(0);
-#pragma unroll
-                        for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                            for (int id = 0; id < ND; ++id) s += ERI_(ia, ib, ic, id) * dbd[ib * ND + id];
-                        atomicAdd(&Kout[(size_t)(oa + ia) * nbf + oc + ic], double(deg * s));
-                    }
-#pragma unroll
-                for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        R s = R(0);
-#pragma unroll
-                        for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                            for (int ic = 0; ic < NC; ++ic) s += ERI_(ia, ib, ic, id) * dbc[ib * NC + ic];
-                        atomicAdd(&Kout[(size_t)(oa + ia) * nbf + od + id], double(deg * s));
-                    }
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                    for (int ic = 0; ic < NC; ++ic) {
-                        R s = R(0);
-#pragma unroll
-                        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-                            for (int id = 0; id < ND; ++id) s += ERI_(ia, ib, ic, id) * dad[ia * ND + id];
-                        atomicAdd(&Kout[(size_t)(ob + ib) * nbf + oc + ic], double(deg * s));
-                    }
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        R s = R(0);
-#pragma unroll
-                        for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-                            for (int ic = 0; ic < NC; ++ic) s += ERI_(ia, ib, ic, id) * dac[ia * NC + ic];
-                        atomicAdd(&Kout[(size_t)(ob + ib) * nbf + od + id], double(deg * s));
-                    }
-#undef ERI_
-            }
+            digest<LA, LB, LC, LD, R, NK>(g, eri, oa, ob, oc, od, deg, dab, jab, [&](int kk, int r, int col, R v) {
+                double *Ko = kk == 0 ? g.K0 : g.K1;
+                const int row = r < NA ? oa + r : ob + r - NA;
+                atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
+            });
         }
     }
-    if constexpr (MODE == MODE_DIRECT || MODE == MODE_LOAD) {
+    if constexpr (MODE == MODE_LOAD) {
 #pragma unroll
         for (int ab = 0; ab < NA * NB; ++ab) {
             const double s = wave_sum(double(jab[ab]));
             if (lane == 0) atomicAdd(&g.J[(size_t)(oa + ab / NB) * nbf + ob + ab % NB], s);
         }
+    }
+    if (g.count) {
+        unsigned long long tot = counted;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
+        if (lane == 0) atomicAdd(g.count, tot);
+    }
+}
+
+// Direct-mode kernel: persistent workgroups take items (bra pair i, block of
+// up to g.chunk of its kets) from a queue laid out by jk_count_kernel /
+// jk_scan_kernel (bra i owns items item_off[i] .. item_off[i+1]-1).  The
+// workgroup's wavefronts split the block; each screens its candidates 64 at
+// a time with the density-weighted test and queues the survivors in LDS, so
+// passes run with full wavefronts.  The bra's exchange rows K'(a|b, :) of
+// every density and its J'_ab are summed in LDS (dynamic shared memory, when
+// g.krows) and flushed to device memory after the item; with g.krows = 0
+// (rows too long for LDS) the exchange terms go straight to device memory.
+template <int LA, int LB, int LC, int LD, typename R, int NK>
+__global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
+    jk_direct_kernel(JKArgs<R> g, const R *__restrict__ bprim, const int *__restrict__ item_off) {
+    constexpr int W = jk_waves<LA, LB, LC, LD>(), BLOCK = 64 * W;
+    using E = Eri<LA, LB, LC, LD, R>;
+    using BT = BoysLds<E::L, R>;
+    constexpr int NA = E::NA, NB = E::NB, NINT = E::NINT, NAB = NA + NB;
+    __shared__ R lds_boys[BT::SIZE];
+    __shared__ int lds_list[W][128];
+    __shared__ int s_item;
+    extern __shared__ double lds_acc[];  // [NK][NAB][nbf] exchange rows if g.krows, then [NA*NB] J'_ab
+    BT::fill(lds_boys, g.tab, threadIdx.x, BLOCK);
+    const PairView<R> &B = g.bra;
+    const PairView<R> &K = g.ket;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nbf = g.nbf;
+    const int nrow = g.krows ? NK * NAB * nbf : 0;
+    double *ljab = lds_acc + nrow;
+    for (int e = threadIdx.x; e < nrow + NA * NB; e += BLOCK) lds_acc[e] = 0.0;
+    int *list = lds_list[wave];
+    const bool compact = g.Dsh != nullptr;
+    const int total = g.work[1];
+    unsigned long long counted = 0;
+    __syncthreads();
+    while (true) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&g.work[0], 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_item);
+        if (t >= total) break;
+        int i = 0, hi = B.n - 1;  // largest i with item_off[i] <= t
+        while (i < hi) {
+            const int mid = (i + hi + 1) >> 1;
+            if (item_off[mid] <= t) i = mid;
+            else hi = mid - 1;
+        }
+        const int jbeg = (t - item_off[i]) * g.chunk, jend = min(g.nket[i], jbeg + g.chunk);
+        // this wavefront's share of the block, a multiple of 64 kets
+        const int per = (((jend - jbeg) + W - 1) / W + 63) & ~63;
+        const int wb = min(jend, jbeg + wave * per), we = min(jend, wb + per);
+
+        const R Qi = B.Q[i];
+        const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
+        const R ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
+        const R deg_ab = (a == b) ? R(0.5) : R(1);
+        R dab[NA * NB], jab[NA * NB];
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                dab[ia * NB + ib] = g.Dj[(size_t)(oa + ia) * nbf + ob + ib];
+                jab[ia * NB + ib] = R(0);
+            }
+
+        int next = wb, queued = 0;  // wavefront-uniform
+        while (true) {
+            int j;
+            if (compact) {
+                const int ns = g.nsh;
+                while (queued < 64 && next < we) {
+                    const int jj = next + lane;
+                    bool keep = false;
+                    if (jj < we) {
+                        const int c = K.sa[jj], d = K.sb[jj];
+                        R dq = R(2) * max(g.Dsh[a * ns + b], g.Dsh[c * ns + d]);
+                        dq = max(dq, max(max(g.Dsh[b * ns + d], g.Dsh[b * ns + c]),
+                                         max(g.Dsh[a * ns + d], g.Dsh[a * ns + c])));
+                        keep = Qi * K.Q[jj] * dq >= g.tau;
+                    }
+                    const unsigned long long m = __ballot(keep);
+                    if (keep) {
+                        const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                        list[queued + pos] = jj;
+                    }
+                    queued += __popcll(m);
+                    next += 64;
+                }
+                // the queue is read across lanes: keep its LDS writes ahead of the reads
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (queued == 0) break;
+                const int take = min(queued, 64);
+                j = lane < take ? list[lane] : -1;
+                // entries 64 .. queued-1 move down; lane l alone reads and writes slot l
+                if (lane + 64 < queued) list[lane] = list[lane + 64];
+                queued -= take;
+            } else {
+                if (next >= we) break;
+                j = next + lane < we ? next + lane : -1;
+                next += 64;
+            }
+            if (j < 0) continue;
+            const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
+            ++counted;
+            R eri[NINT];
+            eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
+            const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
+            if (g.krows) {
+                digest<LA, LB, LC, LD, R, NK>(g, eri, oa, ob, oc, od, deg, dab, jab, [&](int kk, int r, int col, R v) {
+                    atomicAdd(&lds_acc[(kk * NAB + r) * nbf + col], double(v));
+                });
+            } else {
+                digest<LA, LB, LC, LD, R, NK>(g, eri, oa, ob, oc, od, deg, dab, jab, [&](int kk, int r, int col, R v) {
+                    double *Ko = kk == 0 ? g.K0 : g.K1;
+                    const int row = r < NA ? oa + r : ob + r - NA;
+                    atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
+                });
+            }
+        }
+#pragma unroll
+        for (int ab = 0; ab < NA * NB; ++ab) {
+            const double s = wave_sum(double(jab[ab]));
+            if (lane == 0) atomicAdd(&ljab[ab], s);
+        }
+        __syncthreads();
+        // flush the item's rows and J'_ab; entries are left zero for the next item
+        for (int e = threadIdx.x; e < nrow + NA * NB; e += BLOCK) {
+            const double v = lds_acc[e];
+            if (v == 0.0) continue;
+            lds_acc[e] = 0.0;
+            if (e < nrow) {
+                const int kk = e / (NAB * nbf), rc = e - kk * NAB * nbf, r = rc / nbf, col = rc - r * nbf;
+                const int row = r < NA ? oa + r : ob + r - NA;
+                atomicAdd(&(kk == 0 ? g.K0 : g.K1)[(size_t)row * nbf + col], v);
+            } else {
+                const int ab = e - nrow;
+                atomicAdd(&g.J[(size_t)(oa + ab / NB) * nbf + ob + ab % NB], v);
+            }
+        }
+        __syncthreads();
     }
     if (g.count) {
         unsigned long long tot = counted;
@@ -283,10 +447,11 @@ __global__ void __launch_bounds__(64) schwarz_kernel(PairView<R> pv, const R *__
         const R *kq = pv.prim + (size_t)u * PF_NFIELD * pv.npad + i;
         for (int v = 0; v < pv.kp; ++v) {
             const R *bp = pv.prim + (size_t)v * PF_NFIELD * pv.npad + i;
-            E::prim(bp[PF_P * pv.npad], bp[PF_PX * pv.npad], bp[PF_PY * pv.npad], bp[PF_PZ * pv.npad],
-                    bp[PF_AX * pv.npad], bp[PF_AY * pv.npad], bp[PF_AZ * pv.npad], bp[PF_K * pv.npad],
-                    kq[PF_P * pv.npad], kq[PF_PX * pv.npad], kq[PF_PY * pv.npad], kq[PF_PZ * pv.npad],
-                    kq[PF_AX * pv.npad], kq[PF_AY * pv.npad], kq[PF_AZ * pv.npad], kq[PF_K * pv.npad], tab, acc);
+            E::prim(bp[PF_P * pv.npad], bp[PF_O2P * pv.npad], bp[PF_PX * pv.npad], bp[PF_PY * pv.npad],
+                    bp[PF_PZ * pv.npad], bp[PF_AX * pv.npad], bp[PF_AY * pv.npad], bp[PF_AZ * pv.npad],
+                    bp[PF_K * pv.npad], kq[PF_P * pv.npad], kq[PF_O2P * pv.npad], kq[PF_PX * pv.npad],
+                    kq[PF_PY * pv.npad], kq[PF_PZ * pv.npad], kq[PF_AX * pv.npad], kq[PF_AY * pv.npad],
+                    kq[PF_AZ * pv.npad], kq[PF_K * pv.npad], tab, acc);
         }
     }
     R eri[E::NINT];
@@ -299,22 +464,73 @@ __global__ void __launch_bounds__(64) schwarz_kernel(PairView<R> pv, const R *__
     Q[i] = sqrt(m);
 }
 
+// Direct-mode work plan, step 1 (one thread per bra pair i): the ket prefix
+// that can pass Q_i Q_j dmax >= tau (kets sorted by Q descending, binary
+// search; same-class launches stop at j = i) and its number of items.
+template <typename R>
+__global__ void __launch_bounds__(256) jk_count_kernel(JKArgs<R> g) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.bra.n) return;
+    const R qthr = g.tau / (g.bra.Q[i] * (g.dmax ? *g.dmax : R(1)));
+    int lo = 0, hi = g.ket.n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (g.ket.Q[mid] >= qthr) lo = mid + 1;
+        else hi = mid;
+    }
+    const int nk = g.same ? min(lo, i + 1) : lo;
+    g.nket[i] = nk;
+    g.item_off[i] = (nk + g.chunk - 1) / g.chunk;
+}
+
+// Step 2 (one workgroup): exclusive prefix sum of the item counts in place,
+// item_off[n] = total, and the queue head reset (work[0] = 0, work[1] = total).
+__global__ void __launch_bounds__(1024) jk_scan_kernel(int n, int *item_off, int *work);
+
+template <int LA, int LB, int LC, int LD, typename R, int NK>
+void launch_direct(const JKArgs<R> &args, hipStream_t st) {
+    constexpr int W = jk_waves<LA, LB, LC, LD>();
+    constexpr int NA = nc(LA), NB = nc(LB);
+    if (!args.work || !args.item_off || !args.nket) throw Error("jk: direct mode needs its work-plan buffers");
+    JKArgs<R> a = args;
+    a.chunk = jk_chunk<LA, LB, LC, LD>();
+    const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK>;
+    int dev = 0, ncu = 0, maxlds = 0, per = 0;
+    PF_HIP(hipGetDevice(&dev));
+    PF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    PF_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    hipFuncAttributes fa;
+    PF_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern)));
+    const size_t rows = sizeof(double) * (size_t)NK * (NA + NB) * a.nbf, jab = sizeof(double) * NA * NB;
+    a.krows = fa.sharedSizeBytes + rows + jab <= (size_t)maxlds;
+    const size_t dyn = (a.krows ? rows : 0) + jab;
+    if (fa.sharedSizeBytes + dyn > 65536)
+        PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)dyn));
+    PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * W, dyn));
+    const int grid = std::max(1, per) * std::max(1, ncu);  // persistent: as many workgroups as fit at once
+    jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
+    jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.work);
+    kern<<<grid, 64 * W, dyn, st>>>(a, a.bra.prim, a.item_off);
+}
+
 template <int LA, int LB, int LC, int LD, typename R>
 void launch_nk_mode(int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
-    const dim3 grid((args.bra.n + JK_BLOCK / 64 - 1) / (JK_BLOCK / 64)), block(JK_BLOCK);
     if (args.bra.n == 0) return;
+    constexpr int W = jk_waves<LA, LB, LC, LD>();
+    const dim3 grid((args.bra.n + W - 1) / W), block(64 * W);
     if constexpr (std::is_same<R, double>::value) {
-        if (mode == MODE_STORE) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_STORE><<<grid, block, 0, st>>>(args); return; }
-        if (mode == MODE_PACKED) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_PACKED><<<grid, block, 0, st>>>(args); return; }
+        if (mode == MODE_STORE) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_STORE><<<grid, block, 0, st>>>(args, args.bra.prim); return; }
+        if (mode == MODE_PACKED) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_PACKED><<<grid, block, 0, st>>>(args, args.bra.prim); return; }
         if (mode == MODE_LOAD) {
-            if (nk == 1) jk_kernel<LA, LB, LC, LD, R, 1, MODE_LOAD><<<grid, block, 0, st>>>(args);
-            else jk_kernel<LA, LB, LC, LD, R, 2, MODE_LOAD><<<grid, block, 0, st>>>(args);
+            if (nk == 1) jk_kernel<LA, LB, LC, LD, R, 1, MODE_LOAD><<<grid, block, 0, st>>>(args, args.bra.prim);
+            else jk_kernel<LA, LB, LC, LD, R, 2, MODE_LOAD><<<grid, block, 0, st>>>(args, args.bra.prim);
             return;
         }
     }
     if (mode != MODE_DIRECT) throw Error("jk: mode not available in this precision");
-    if (nk == 1) jk_kernel<LA, LB, LC, LD, R, 1, MODE_DIRECT><<<grid, block, 0, st>>>(args);
-    else jk_kernel<LA, LB, LC, LD, R, 2, MODE_DIRECT><<<grid, block, 0, st>>>(args);
+    if (nk == 1) launch_direct<LA, LB, LC, LD, R, 1>(args, st);
+    else launch_direct<LA, LB, LC, LD, R, 2>(args, st);
 }
 
 }  // namespace pf

@@ -52,11 +52,12 @@ __device__ __forceinline__ void contracted_block(const double *__restrict__ bp, 
         const double *q = kq + (size_t)u * PF_NFIELD * knpad + kj;
         const double qq = q[PF_P * knpad], Qx = q[PF_PX * knpad], Qy = q[PF_PY * knpad], Qz = q[PF_PZ * knpad];
         const double QCx = q[PF_AX * knpad], QCy = q[PF_AY * knpad], QCz = q[PF_AZ * knpad], Kk = q[PF_K * knpad];
+        const double o2q = q[PF_O2P * knpad];
         for (int v = 0; v < kpb; ++v) {
             const double *b = bp + (size_t)v * PF_NFIELD * bnpad + bi;
-            E::prim(b[PF_P * bnpad], b[PF_PX * bnpad], b[PF_PY * bnpad], b[PF_PZ * bnpad], b[PF_AX * bnpad],
-                    b[PF_AY * bnpad], b[PF_AZ * bnpad], b[PF_K * bnpad], qq, Qx, Qy, Qz, QCx, QCy, QCz, Kk, tab,
-                    acc);
+            E::prim(b[PF_P * bnpad], b[PF_O2P * bnpad], b[PF_PX * bnpad], b[PF_PY * bnpad], b[PF_PZ * bnpad],
+                    b[PF_AX * bnpad], b[PF_AY * bnpad], b[PF_AZ * bnpad], b[PF_K * bnpad], qq, o2q, Qx, Qy, Qz, QCx,
+                    QCy, QCz, Kk, tab, acc);
         }
     }
     E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);

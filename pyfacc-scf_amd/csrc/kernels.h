@@ -12,10 +12,10 @@ void launch_onee(const Shell *d_shells, int nsh, int nbf, int natoms, const doub
 
 // ---------------------------------------------------------------- pairs
 // Per primitive pair data, in this field order, stored structure-of-arrays.
-enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_NFIELD };
+enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_O2P, PF_NFIELD };
 // PF_AX.. hold P - A (A = first shell of the pair); PF_K holds
 // sqrt(2) pi^{5/4} c_a c_b exp(-ab/p |AB|^2) / p, so that the quartet
-// prefactor is K_bra K_ket / sqrt(p + q).
+// prefactor is K_bra K_ket / sqrt(p + q); PF_O2P holds 1/(2p).
 
 // Device view of one shell-pair class (LA >= LB), pairs sorted by Schwarz
 // bound Q descending.  Arrays are structure-of-arrays with stride npad.
@@ -26,7 +26,8 @@ struct PairView {
     const int *oa = nullptr, *ob = nullptr;  // first basis function of each shell
     const R *Q = nullptr;                    // [n] Schwarz bound
     const R *AB = nullptr;                   // [3][npad]  A - B
-    const R *prim = nullptr;                 // [kp][PF_NFIELD][npad]
+    const R *prim = nullptr;                 // [kp][PF_NFIELD][npad], per pair by bound |K|/sqrt(p) descending
+    const int *kpn = nullptr;                // [npad] primitive pairs that the ERI kernels visit
 };
 
 // Work description shared by all ERI/J/K kernels of one launch.
@@ -53,6 +54,11 @@ struct JKArgs {
     double *packed = nullptr;
     // statistics (quartets that passed screening), may be nullptr
     unsigned long long *count = nullptr;
+    // direct-mode work plan (device): per bra ket-prefix length [bra.n],
+    // first work item [bra.n + 1], queue head and item total [2]; kets per item
+    int *nket = nullptr, *item_off = nullptr, *work = nullptr;
+    int chunk = 0;
+    int krows = 0;  // direct mode: bra exchange rows summed in LDS
 };
 
 enum JKMode { MODE_DIRECT = 0, MODE_STORE = 1, MODE_LOAD = 2, MODE_PACKED = 3 };
