@@ -239,7 +239,7 @@ int pf_projector(int n, const double *F, double nocc, double *P) {
         hipStream_t st;
         PF_HIP(hipStreamCreate(&st));
         {
-            DevBuf<double> dF((size_t)n * n), dP((size_t)n * n), w((size_t)n * n), state(8);
+            DevBuf<double> dF((size_t)n * n), dP((size_t)n * n), w((size_t)n * n), state(sp2_state_size(n));
             dF.upload(F, (size_t)n * n, st);
             sp2_density(n, dF.p, nocc, dP.p, w.p, state.p, st);
             dP.download(P, (size_t)n * n, st);

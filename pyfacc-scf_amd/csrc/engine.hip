@@ -455,10 +455,12 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         throw Error("stored ERIs are FP64 only");
 
     setup_one_electron();
-    w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); state_.alloc(8);
-    // X = U s^{-1/2} (la.f90:36-50), computed once
+    w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); state_.alloc(sp2_state_size(n));
+    // orthogonaliser with X X^T = S^{-1}, computed once: S^{-1/2} by Newton-Schulz,
+    // or the reference's X = U s^{-1/2} (la.f90:36-50) from the Jacobi solver
+    // when that iteration does not converge
     X_.alloc(n2);
-    {
+    if (!inv_sqrt_spd(n, S_.p, X_.p, &res.min_overlap_eig, st_)) {
         DevBuf<double> s(n), U(n2);
         jacobi_eigh(n, S_.p, s.p, U.p, st_);
         std::vector<double> hs(n);
@@ -478,6 +480,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     const int nocc[2] = {uhf ? nalpha : nalpha, nbeta};
     const double occ = uhf ? 1.0 : 2.0;
     DevBuf<double> D(n2 * nd), Dold(n2 * nd), Dbuilt(n2 * nd), dD(n2 * nd), G(n2 * nd), F(n2 * nd), red(4);
+    DevBuf<double> part((size_t)RED_BLOCKS * DotPtrs::MAX);  // reduction partials
     // core guess (rscf.f90:113-114; uscf.f90:140-144, C_alpha = C_beta)
     {
         const double td = now();
@@ -552,7 +555,12 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
             PF_HIP(hipMemcpyAsync(dF[slot].p, F.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
             head = (head + 1) % ndiis;
             if (nstore < ndiis) ++nstore;
-            for (int k = 0; k < nstore; ++k) dev_dot(n2 * nd, dE[slot].p, dE[k].p, dots.p + k, st_);
+            for (int k0 = 0; k0 < nstore; k0 += DotPtrs::MAX) {  // B-matrix row, up to 8 dots per launch
+                const int m = std::min(DotPtrs::MAX, nstore - k0);
+                DotPtrs vecs{};
+                for (int k = 0; k < m; ++k) vecs.p[k] = dE[k0 + k].p;
+                dev_dots(n2 * nd, dE[slot].p, vecs, m, dots.p + k0, part.p, st_);
+            }
             std::vector<double> row(nstore);
             dots.download(row.data(), nstore, st_);
             PF_HIP(hipStreamSynchronize(st_));
@@ -596,9 +604,9 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         }
         for (int s = 0; s < nd; ++s) density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr);
         // energy with the new D and the F built this iteration (rscf.f90:144-151; uscf.f90:186-193)
-        if (!uhf) dev_energy_rhf(n, D.p, H_.p, F.p, red.p, st_);
-        else dev_energy_uhf(n, D.p, D.p + n2, H_.p, F.p, F.p + n2, red.p, st_);
-        dev_diff_sq(n2 * nd, D.p, Dold.p, red.p + 1, st_);
+        if (!uhf) dev_energy_rhf(n, D.p, H_.p, F.p, red.p, part.p, st_);
+        else dev_energy_uhf(n, D.p, D.p + n2, H_.p, F.p, F.p + n2, red.p, part.p, st_);
+        dev_diff_sq(n2 * nd, D.p, Dold.p, red.p + 1, part.p, st_);
         double hv[2];
         PF_HIP(hipMemcpyAsync(hv, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
         PF_HIP(hipStreamSynchronize(st_));

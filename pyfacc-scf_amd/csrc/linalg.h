@@ -6,12 +6,20 @@
 
 namespace pf {
 
-void dev_dot(int64_t n, const double *a, const double *b, double *out, hipStream_t st);
-void dev_diff_sq(int64_t n, const double *a, const double *b, double *out, hipStream_t st);
-void dev_trace(int n, const double *a, double *out, hipStream_t st);
-void dev_energy_rhf(int n, const double *D, const double *H, const double *F, double *out, hipStream_t st);
+// Reductions to device scalars; part is scratch of RED_BLOCKS doubles per
+// result (the sums are deterministic: fixed partition and order).
+constexpr int RED_BLOCKS = 256;
+struct DotPtrs {
+    static constexpr int MAX = 8;
+    const double *p[MAX];
+};
+// out[y] = a . b.p[y], y < m <= 8
+void dev_dots(int64_t n, const double *a, const DotPtrs &b, int m, double *out, double *part, hipStream_t st);
+void dev_diff_sq(int64_t n, const double *a, const double *b, double *out, double *part, hipStream_t st);
+void dev_energy_rhf(int n, const double *D, const double *H, const double *F, double *out, double *part,
+                    hipStream_t st);
 void dev_energy_uhf(int n, const double *Da, const double *Db, const double *H, const double *Fa, const double *Fb,
-                    double *out, hipStream_t st);
+                    double *out, double *part, hipStream_t st);
 void dev_fill(int64_t n, double v, double *x, hipStream_t st);
 void dev_fillf(int64_t n, float v, float *x, hipStream_t st);
 template <typename R>
@@ -28,8 +36,14 @@ void dev_shell_dmax(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const 
 void jacobi_eigh(int n, const double *A, double *w, double *V, hipStream_t st, int max_sweeps = 30,
                  double tol = 1e-15);
 // Spectral projector onto the nocc lowest eigenvectors of symmetric Fp;
-// X receives the projector, work is n x n scratch, state holds 6 doubles.
-// Returns the number of iterations used.
+// X receives the projector, work is n x n scratch, state holds
+// sp2_state_size(n) doubles.  Returns the number of iterations used.
+// S^{-1/2} of a symmetric positive definite S (coupled Newton-Schulz on FP64
+// MFMA GEMMs) and an estimate of lambda_min(S) (relative error < 1e-6).
+// Returns false, leaving X undefined, when the iteration has not converged
+// (S singular or nearly so); synchronises the stream.
+bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_t st);
+size_t sp2_state_size(int n);
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
                 int max_iter = 100);
 

@@ -10,7 +10,8 @@
 //     orbital energies;
 //   * SP2 spectral projector (Niklasson, PRB 66, 155115 (2002)): the density
 //     of the lowest n_occ eigenvectors of F' = X^T F X built from GEMMs only,
-//     X <- X^2 or 2X - X^2 chosen on the device from the two traces;
+//     X <- X^2 or 2X - X^2 chosen on the device from the two traces, one
+//     kernel per step (the traces of the next iterate come from its epilogue);
 //   * reductions (energy, ||dD||_F, traces, dot products) and element-wise
 //     Fock assembly.
 #include <algorithm>
@@ -39,15 +40,13 @@ __device__ __forceinline__ double ld_op(const double *__restrict__ M, int ld, in
     return t ? M[(size_t)c * ld + r] : M[(size_t)r * ld + c];
 }
 
-__global__ void __launch_bounds__(256) gemm_f64_kernel(int m, int n, int k, int ta, int tb, double alpha,
-                                                        const double *__restrict__ alpha_p,
-                                                        const double *__restrict__ A, int lda,
-                                                        const double *__restrict__ B, int ldb, double beta,
-                                                        const double *__restrict__ beta_p, const double *Cin,
-                                                        double *C, int ldc) {
-    __shared__ double red[4][16][64];
+// The 32x32 tile (r0, c0) of op(A) op(B), summed over the four wavefronts'
+// K slices in red; afterwards thread t holds tile elements idx = t + 256 q
+// (q = 0..3) as red[0..3][reg][ln] summed, reg = idx >> 6, ln = idx & 63.
+__device__ __forceinline__ void mfma_tile(int m, int n, int k, int ta, int tb, const double *__restrict__ A, int lda,
+                                          const double *__restrict__ B, int ldb, int r0, int c0,
+                                          double (*red)[16][64]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
     const int i = lane & 15, kq = lane >> 4;
     double4_t acc[2][2];
 #pragma unroll
@@ -72,15 +71,33 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(int m, int n, int k, int 
 #pragma unroll
             for (int r = 0; r < 4; ++r) red[w][(a * 2 + b) * 4 + r][lane] = acc[a][b][r];
     __syncthreads();
+}
+
+// Row and column in C of tile element idx (0..1023) of the tile at (r0, c0).
+// D layout of the MFMA: register r of lane l holds row (l >> 4) + 4 r, column l & 15.
+__device__ __forceinline__ void tile_pos(int idx, int r0, int c0, int &gr, int &gc) {
+    const int reg = idx >> 6, ln = idx & 63;
+    const int a = reg >> 3, b = (reg >> 2) & 1, r = reg & 3;
+    gr = r0 + 16 * a + (ln >> 4) + 4 * r;
+    gc = c0 + 16 * b + (ln & 15);
+}
+
+__global__ void __launch_bounds__(256) gemm_f64_kernel(int m, int n, int k, int ta, int tb, double alpha,
+                                                        const double *__restrict__ alpha_p,
+                                                        const double *__restrict__ A, int lda,
+                                                        const double *__restrict__ B, int ldb, double beta,
+                                                        const double *__restrict__ beta_p, const double *Cin,
+                                                        double *C, int ldc) {
+    __shared__ double red[4][16][64];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    mfma_tile(m, n, k, ta, tb, A, lda, B, ldb, r0, c0, red);
     const double al = alpha_p ? *alpha_p : alpha, be = beta_p ? *beta_p : beta;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int idx = threadIdx.x + 256 * q;  // 0..1023 = (reg, lane)
-        const int reg = idx >> 6, ln = idx & 63;
+        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
         const double v = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
-        // D layout: register r of lane l holds row (l >> 4) + 4 r, column l & 15
-        const int a = reg >> 3, b = (reg >> 2) & 1, r = reg & 3;
-        const int gr = r0 + 16 * a + (ln >> 4) + 4 * r, gc = c0 + 16 * b + (ln & 15);
+        int gr, gc;
+        tile_pos(idx, r0, c0, gr, gc);
         if (gr < m && gc < n) {
             const size_t o = (size_t)gr * ldc + gc;
             C[o] = be == 0.0 ? al * v : al * v + be * Cin[o];
@@ -108,7 +125,80 @@ void gemm_f64(int n, bool ta, bool tb, double alpha, const double *A, const doub
 }
 
 // ------------------------------------------------------------------ reductions
-// out[slot] = sum_i f(i); single block of 1024 threads (n <= a few million).
+// Block sum of one value per thread (256 threads); every thread gets the sum.
+__device__ __forceinline__ double block_sum256(double v, double *sh) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double t = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return t;
+}
+
+// Two-stage deterministic reductions: up to RED_BLOCKS workgroups write one
+// partial sum each (part[y * RED_BLOCKS + block] for result y), then one
+// workgroup per result adds the partials in a fixed order.
+template <typename Fn>
+__global__ void __launch_bounds__(256) reduce_part_kernel(int64_t n, Fn f, double *part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += f(i, blockIdx.y);
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * RED_BLOCKS + blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(256) reduce_final_kernel(int nb, const double *part, double *out) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += part[(size_t)blockIdx.x * RED_BLOCKS + b];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+// out[y] = sum_i f(i, y), y < ny
+template <typename Fn>
+static void reduce2(int64_t n, int ny, Fn f, double *out, double *part, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(RED_BLOCKS, std::max<int64_t>(1, (n + 1023) / 1024));
+    reduce_part_kernel<<<dim3(nb, ny), 256, 0, st>>>(n, f, part);
+    reduce_final_kernel<<<ny, 256, 0, st>>>(nb, part, out);
+    PF_HIP(hipGetLastError());
+}
+
+struct DotsF {  // a . b_y
+    const double *a;
+    DotPtrs b;
+    __device__ double operator()(int64_t i, int y) const { return a[i] * b.p[y][i]; }
+};
+struct DiffSqF {
+    const double *a, *b;
+    __device__ double operator()(int64_t i, int) const { const double d = a[i] - b[i]; return d * d; }
+};
+struct EnergyRF {  // D (H + F)
+    const double *D, *H, *F;
+    __device__ double operator()(int64_t i, int) const { return D[i] * (H[i] + F[i]); }
+};
+struct EnergyUF {  // 1/2 (Da+Db) H + 1/2 (Da Fa + Db Fb)
+    const double *Da, *Db, *H, *Fa, *Fb;
+    __device__ double operator()(int64_t i, int) const {
+        return 0.5 * (Da[i] + Db[i]) * H[i] + 0.5 * (Da[i] * Fa[i] + Db[i] * Fb[i]);
+    }
+};
+
+void dev_dots(int64_t n, const double *a, const DotPtrs &b, int m, double *out, double *part, hipStream_t st) {
+    if (m < 1 || m > DotPtrs::MAX) throw Error("dev_dots: 1..8 vectors");
+    reduce2(n, m, DotsF{a, b}, out, part, st);
+}
+void dev_diff_sq(int64_t n, const double *a, const double *b, double *out, double *part, hipStream_t st) {
+    reduce2(n, 1, DiffSqF{a, b}, out, part, st);
+}
+void dev_energy_rhf(int n, const double *D, const double *H, const double *F, double *out, double *part,
+                    hipStream_t st) {
+    reduce2((int64_t)n * n, 1, EnergyRF{D, H, F}, out, part, st);
+}
+void dev_energy_uhf(int n, const double *Da, const double *Db, const double *H, const double *Fa, const double *Fb,
+                    double *out, double *part, hipStream_t st) {
+    reduce2((int64_t)n * n, 1, EnergyUF{Da, Db, H, Fa, Fb}, out, part, st);
+}
+
+// Single-workgroup reduction for the Jacobi solver's small helper sums.
 template <typename Fn>
 __global__ void __launch_bounds__(1024) reduce_kernel(int64_t n, Fn f, double *out) {
     __shared__ double part[16];
@@ -122,52 +212,6 @@ __global__ void __launch_bounds__(1024) reduce_kernel(int64_t n, Fn f, double *o
         for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
         if (threadIdx.x == 0) *out = s;
     }
-}
-
-struct DotF {
-    const double *a, *b;
-    __device__ double operator()(int64_t i) const { return a[i] * b[i]; }
-};
-struct DiffSqF {
-    const double *a, *b;
-    __device__ double operator()(int64_t i) const { const double d = a[i] - b[i]; return d * d; }
-};
-struct TraceF {
-    const double *a;
-    int n;
-    __device__ double operator()(int64_t i) const { return a[i * (n + 1)]; }
-};
-struct EnergyRF {  // D (H + F)
-    const double *D, *H, *F;
-    __device__ double operator()(int64_t i) const { return D[i] * (H[i] + F[i]); }
-};
-struct EnergyUF {  // 1/2 (Da+Db) H + 1/2 (Da Fa + Db Fb)
-    const double *Da, *Db, *H, *Fa, *Fb;
-    __device__ double operator()(int64_t i) const {
-        return 0.5 * (Da[i] + Db[i]) * H[i] + 0.5 * (Da[i] * Fa[i] + Db[i] * Fb[i]);
-    }
-};
-
-void dev_dot(int64_t n, const double *a, const double *b, double *out, hipStream_t st) {
-    reduce_kernel<<<1, 1024, 0, st>>>(n, DotF{a, b}, out);
-    PF_HIP(hipGetLastError());
-}
-void dev_diff_sq(int64_t n, const double *a, const double *b, double *out, hipStream_t st) {
-    reduce_kernel<<<1, 1024, 0, st>>>(n, DiffSqF{a, b}, out);
-    PF_HIP(hipGetLastError());
-}
-void dev_trace(int n, const double *a, double *out, hipStream_t st) {
-    reduce_kernel<<<1, 1024, 0, st>>>(n, TraceF{a, n}, out);
-    PF_HIP(hipGetLastError());
-}
-void dev_energy_rhf(int n, const double *D, const double *H, const double *F, double *out, hipStream_t st) {
-    reduce_kernel<<<1, 1024, 0, st>>>((int64_t)n * n, EnergyRF{D, H, F}, out);
-    PF_HIP(hipGetLastError());
-}
-void dev_energy_uhf(int n, const double *Da, const double *Db, const double *H, const double *Fa, const double *Fb,
-                    double *out, hipStream_t st) {
-    reduce_kernel<<<1, 1024, 0, st>>>((int64_t)n * n, EnergyUF{Da, Db, H, Fa, Fb}, out);
-    PF_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ element-wise
@@ -440,18 +484,27 @@ void jacobi_eigh(int n, const double *A_in, double *w, double *V, hipStream_t st
     PF_HIP(hipStreamSynchronize(st));
 }
 
-// ------------------------------------------------------------------ SP2
-// Gershgorin bounds of a symmetric matrix -> bounds[0] = min, bounds[1] = max.
-__global__ void __launch_bounds__(1024) gershgorin_kernel(int n, const double *A, double *bounds) {
+// Gershgorin discs, one wavefront per row: rl[i] = A_ii - r_i, rh[i] = A_ii + r_i.
+__global__ void __launch_bounds__(256) gershgorin_rows_kernel(int n, const double *A, double *rl, double *rh) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    double r = 0.0;
+    for (int j = lane; j < n; j += 64)
+        if (j != i) r += fabs(A[(size_t)i * n + j]);
+    for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
+    if (lane == 0) {
+        const double d = A[(size_t)i * n + i];
+        rl[i] = d - r;
+        rh[i] = d + r;
+    }
+}
+__global__ void __launch_bounds__(1024) gershgorin_bounds_kernel(int n, const double *rl, const double *rh,
+                                                                 double *bounds) {
     __shared__ double lo_s[16], hi_s[16];
     double lo = 1e300, hi = -1e300;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double r = 0.0;
-        for (int j = 0; j < n; ++j)
-            if (j != i) r += fabs(A[(size_t)i * n + j]);
-        const double d = A[(size_t)i * n + i];
-        lo = fmin(lo, d - r);
-        hi = fmax(hi, d + r);
+        lo = fmin(lo, rl[i]);
+        hi = fmax(hi, rh[i]);
     }
     for (int o = 32; o >= 1; o >>= 1) {
         lo = fmin(lo, __shfl_xor(lo, o, 64));
@@ -460,79 +513,244 @@ __global__ void __launch_bounds__(1024) gershgorin_kernel(int n, const double *A
     if ((threadIdx.x & 63) == 0) { lo_s[threadIdx.x >> 6] = lo; hi_s[threadIdx.x >> 6] = hi; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { lo = fmin(lo, lo_s[w]); hi = fmax(hi, hi_s[w]); }
-        lo = fmin(lo, lo_s[0]);
-        hi = fmax(hi, hi_s[0]);
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { lo = fmin(lo, lo_s[w]); hi = fmax(hi, hi_s[w]); }
         bounds[0] = lo;
         bounds[1] = hi;
     }
 }
-// X = (hi I - F) / (hi - lo)
-__global__ void sp2_init_kernel(int n, const double *F, const double *bounds, double *X) {
+
+// ------------------------------------------------------------------ S^{-1/2}
+// Coupled Newton-Schulz iteration (Higham, Numer. Algorithms 15 (1997) 227):
+// Y0 = S/c, Z0 = I, T = (3I - Z Y)/2, Y <- Y T, Z <- T Z, with c >= lambda_max(S)
+// from Gershgorin discs, converges quadratically to Y = (S/c)^{1/2},
+// Z = (S/c)^{-1/2} for positive definite S; X = Z / sqrt(c) = S^{-1/2}.
+// X X^T = S^{-1} as for the reference's canonical U s^{-1/2} (la.f90:36-50),
+// so densities and orbital energies are the same.
+__global__ void ns_init_kernel(int n, const double *S, const double *bounds, double *Y, double *Z, double *I) {
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const double lo = bounds[0], hi = bounds[1];
-    const double s = 1.0 / (hi - lo);
-    X[(size_t)i * n + j] = ((i == j ? hi : 0.0) - F[(size_t)i * n + j]) * s;
+    const size_t o = (size_t)i * n + j;
+    Y[o] = S[o] / bounds[1];
+    Z[o] = I[o] = (i == j) ? 1.0 : 0.0;
 }
-// One block: t1 = tr X, t2 = sum X_ij^2 (= tr X^2 for symmetric X).  Chooses
-// the next step X <- X^2 (alpha 1, beta 0) when |t2 - nocc| < |2 t1 - t2 - nocc|,
-// else X <- 2X - X^2 (alpha -1, beta 2); once the idempotency error t1 - t2 is
-// below tol (or stops shrinking below 1e-8) the step becomes X <- X
-// (alpha 0, beta 1) and state[2] flags completion.
-// state: 0 t1, 1 t2, 2 done, 3 previous error, 4 alpha, 5 beta, 6/7 bounds
-__global__ void __launch_bounds__(1024) sp2_step_kernel(int n, double nocc, const double *X, double *state,
-                                                         int iter, double tol) {
-    __shared__ double p1[16], p2[16];
-    double t1 = 0.0, t2 = 0.0;
-    const size_t n2 = (size_t)n * n;
-    for (size_t e = threadIdx.x; e < n2; e += blockDim.x) {
-        const double v = X[e];
-        t2 += v * v;
-        if (e % (n + 1) == 0) t1 += v;
+__global__ void ns_finish_kernel(int n, const double *Z, const double *bounds, double *X) {
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    X[(size_t)i * n + j] = Z[(size_t)i * n + j] / sqrt(bounds[1]);
+}
+// out = max_ij |T_ij - delta_ij|
+__global__ void __launch_bounds__(1024) resid_kernel(int n, const double *T, double *out) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += 1024) {
+        const int64_t i = e / n, j = e - i * n;
+        m = fmax(m, fabs(T[e] - (i == j ? 1.0 : 0.0)));
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        t1 += __shfl_xor(t1, o, 64);
-        t2 += __shfl_xor(t2, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) { p1[threadIdx.x >> 6] = t1; p2[threadIdx.x >> 6] = t2; }
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        t1 = 0.0; t2 = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { t1 += p1[w]; t2 += p2[w]; }
-        state[0] = t1;
-        state[1] = t2;
-        const double err = fabs(t1 - t2);
-        if (state[2] == 0.0 && iter > 2 && (err < tol || (err < 1e-8 && err >= state[3]))) state[2] = 1.0;
-        state[3] = err;
-        if (state[2] != 0.0) { state[4] = 0.0; state[5] = 1.0; }
-        else if (fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc)) { state[4] = 1.0; state[5] = 0.0; }
-        else { state[4] = -1.0; state[5] = 2.0; }
+        for (int w = 0; w < 16; ++w) m = fmax(m, sh[w]);
+        *out = m;
+    }
+}
+// Repeated squaring M <- M^2 / tr(M)^2 for lambda_max: t = tr M, alpha = 1/t^2, logt = log t.
+__global__ void __launch_bounds__(1024) trace_log_kernel(int n, const double *M, double *alpha, double *logt) {
+    __shared__ double sh[16];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) t += M[(size_t)i * n + i];
+    for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        t = 0.0;
+        for (int w = 0; w < 16; ++w) t += sh[w];
+        *alpha = 1.0 / (t * t);
+        *logt = log(t);
+    }
+}
+
+bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_t st) {
+    const size_t n2 = (size_t)n * n;
+    DevBuf<double> buf(6 * n2), small(2 * (size_t)n + 64);
+    double *Y = buf.p, *Z = Y + n2, *T = Z + n2, *Y2 = T + n2, *Z2 = Y2 + n2, *I = Z2 + n2;
+    double *bounds = small.p, *resid = small.p + 2, *alpha = small.p + 3, *logs = small.p + 8, *rows = small.p + 64;
+    gershgorin_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, S, rows, rows + n);
+    gershgorin_bounds_kernel<<<1, 1024, 0, st>>>(n, rows, rows + n, bounds);
+    const dim3 g2((n + 127) / 128, n);
+    ns_init_kernel<<<g2, 128, 0, st>>>(n, S, bounds, Y, Z, I);
+    bool ok = false;
+    for (int round = 0; round < 6 && !ok; ++round) {
+        for (int k = 0; k < 8; ++k) {
+            gemm_f64_ex(n, n, n, false, false, -0.5, nullptr, Z, n, Y, n, 1.5, nullptr, I, T, n, st);  // (3I - ZY)/2
+            gemm_f64_ex(n, n, n, false, false, 1.0, nullptr, Y, n, T, n, 0.0, nullptr, Y2, Y2, n, st);
+            gemm_f64_ex(n, n, n, false, false, 1.0, nullptr, T, n, Z, n, 0.0, nullptr, Z2, Z2, n, st);
+            std::swap(Y, Y2);
+            std::swap(Z, Z2);
+        }
+        gemm_f64_ex(n, n, n, false, false, 1.0, nullptr, Z, n, Y, n, 0.0, nullptr, T, T, n, st);
+        resid_kernel<<<1, 1024, 0, st>>>(n, T, resid);
+        double r = 0.0;
+        PF_HIP(hipMemcpyAsync(&r, resid, sizeof(double), hipMemcpyDeviceToHost, st));
+        PF_HIP(hipStreamSynchronize(st));
+        ok = std::isfinite(r) && r < 1e-11;
+    }
+    if (!ok) return false;
+    ns_finish_kernel<<<g2, 128, 0, st>>>(n, Z, bounds, X);
+    // lambda_min(S) = c / lambda_max(Z)^2, lambda_max(Z) = lim tr(Z^(2^k))^(2^-k): with
+    // M_{k+1} = M_k^2 / t_k^2, t_k = tr M_k, log tr Z^(2^K) = log P_K + log t_K where
+    // log P_{k+1} = 2 (log P_k + log t_k); relative error below n^(2^-K) - 1.
+    constexpr int KSQ = 24;
+    static_assert(8 + KSQ + 1 <= 64, "log slots");
+    PF_HIP(hipMemcpyAsync(T, Z, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
+    double *M = T, *M2 = Y2;
+    for (int k = 0; k < KSQ; ++k) {
+        trace_log_kernel<<<1, 1024, 0, st>>>(n, M, alpha, logs + k);
+        gemm_f64_ex(n, n, n, false, false, 0.0, alpha, M, n, M, n, 0.0, nullptr, M2, M2, n, st);
+        std::swap(M, M2);
+    }
+    trace_log_kernel<<<1, 1024, 0, st>>>(n, M, alpha, logs + KSQ);
+    double hl[KSQ + 1], c[2];
+    PF_HIP(hipMemcpyAsync(hl, logs, sizeof(hl), hipMemcpyDeviceToHost, st));
+    PF_HIP(hipMemcpyAsync(c, bounds, sizeof(c), hipMemcpyDeviceToHost, st));
+    PF_HIP(hipStreamSynchronize(st));
+    double logP = 0.0;
+    for (int k = 0; k < KSQ; ++k) logP = 2.0 * (logP + hl[k]);
+    const double log_lmax_z = (logP + hl[KSQ]) / std::ldexp(1.0, KSQ);
+    *lam_min = c[1] * std::exp(-2.0 * log_lmax_z);
+    PF_HIP(hipGetLastError());
+    return true;
+}
+
+// ------------------------------------------------------------------ SP2
+// Work layout (sp2_state_size doubles): [0] lo, [1] hi Gershgorin bounds;
+// [2 + 2s] idempotency error and [3 + 2s] steps since convergence (0: not
+// converged) recorded with slot s;
+// from [8]: per-tile partial sums P[s][q][tile], q = 0 trace, q = 1 sum of
+// squares, for three rotating slots s; then 2n doubles of Gershgorin row
+// bounds.  Step k reads slot k % 3 (written by
+// step k-1, or by the initial kernel for k = 0) and writes slot (k+1) % 3, so
+// one kernel per step needs no separate reduction launch, and every tile sums
+// the partials in the same order (the decisions are deterministic).
+static int sp2_tiles(int n) { return ((n + 31) / 32) * ((n + 31) / 32); }
+size_t sp2_state_size(int n) { return 8 + 6 * (size_t)sp2_tiles(n) + 2 * (size_t)n; }
+
+// X0 = (hi I - F) / (hi - lo) on 32x32 tiles, with slot-0 partials and records.
+__global__ void __launch_bounds__(256) sp2_init_kernel(int n, const double *F, double *state, double *X) {
+    __shared__ double sh[4];
+    const int nt = (n + 31) / 32, tile = blockIdx.y * nt + blockIdx.x, ntile = nt * nt;
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const double lo = state[0], hi = state[1], sc = 1.0 / (hi - lo);
+    double t1 = 0.0, t2 = 0.0;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int r = r0 + (e >> 5), c = c0 + (e & 31);
+        if (r < n && c < n) {
+            const double v = ((r == c ? hi : 0.0) - F[(size_t)r * n + c]) * sc;
+            X[(size_t)r * n + c] = v;
+            t2 += v * v;
+            if (r == c) t1 += v;
+        }
+    }
+    t1 = block_sum256(t1, sh);
+    t2 = block_sum256(t2, sh);
+    if (threadIdx.x == 0) {
+        state[8 + 0 * ntile + tile] = t1;
+        state[8 + 1 * ntile + tile] = t2;
+        if (tile == 0) { state[2] = 1e300; state[3] = 0.0; }
+    }
+}
+
+// One purification step k: from slot k % 3 (t1 = tr X, t2 = sum X_ij^2 = tr X^2
+// for symmetric X, the previous error and done flag), X' = X^2 when
+// |t2 - nocc| < |2 t1 - t2 - nocc|, else 2X - X^2; once the idempotency error
+// |t1 - t2| is below tol (or stops shrinking below 1e-8, after step 2) the
+// step copies X and the done flag is raised.  X' partials and records go to
+// slot (k+1) % 3.
+__global__ void __launch_bounds__(256) sp2_step_kernel(int n, double nocc, double tol, int k, const double *X,
+                                                        double *Xo, double *state) {
+    __shared__ double red[4][16][64];
+    __shared__ double sh[4];
+    const int nt = (n + 31) / 32, tile = blockIdx.y * nt + blockIdx.x, ntile = nt * nt;
+    const int s0 = k % 3, s1 = (k + 1) % 3;
+    // record [3 + 2s]: steps since convergence (0 = not converged).  The first
+    // two converged steps copy X into both buffers; later ones do nothing.
+    const double age = state[3 + 2 * s0];
+    if (age >= 2.0) {
+        if (tile == 0 && threadIdx.x == 0) {
+            state[2 + 2 * s1] = state[2 + 2 * s0];
+            state[3 + 2 * s1] = age + 1.0;
+        }
+        return;
+    }
+    const double *P = state + 8;
+    double t1 = 0.0, t2 = 0.0;
+    for (int b = threadIdx.x; b < ntile; b += 256) {  // fixed order: deterministic sums
+        t1 += P[(2 * s0 + 0) * ntile + b];
+        t2 += P[(2 * s0 + 1) * ntile + b];
+    }
+    t1 = block_sum256(t1, sh);
+    t2 = block_sum256(t2, sh);
+    const double err = fabs(t1 - t2), prev = state[2 + 2 * s0];
+    bool done = age != 0.0;
+    if (!done && k > 2 && (err < tol || (err < 1e-8 && err >= prev))) done = true;
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
+    if (!done) mfma_tile(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
+    double u1 = 0.0, u2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
+        int gr, gc;
+        tile_pos(idx, r0, c0, gr, gc);
+        if (gr < n && gc < n) {
+            const size_t o = (size_t)gr * n + gc;
+            double v;
+            if (done) {
+                v = X[o];
+            } else {
+                const double x2 = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+                v = square ? x2 : 2.0 * X[o] - x2;
+            }
+            Xo[o] = v;
+            u2 += v * v;
+            if (gr == gc) u1 += v;
+        }
+    }
+    u1 = block_sum256(u1, sh);
+    u2 = block_sum256(u2, sh);
+    if (threadIdx.x == 0) {
+        state[8 + (2 * s1 + 0) * ntile + tile] = u1;
+        state[8 + (2 * s1 + 1) * ntile + tile] = u2;
+        if (tile == 0) {
+            state[2 + 2 * s1] = err;
+            state[3 + 2 * s1] = done ? age + 1.0 : 0.0;
+        }
     }
 }
 
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
                 int max_iter) {
-    dim3 g2((n + 127) / 128, n);
-    gershgorin_kernel<<<1, 1024, 0, st>>>(n, Fp, state + 6);
-    sp2_init_kernel<<<g2, 128, 0, st>>>(n, Fp, state + 6, X);
-    dev_fill(3, 0.0, state, st);
-    dev_fill(1, 1e300, state + 3, st);
+    const int nt = (n + 31) / 32;
+    const dim3 tiles(nt, nt);
+    double *rows = state + 8 + 6 * (size_t)sp2_tiles(n);
+    gershgorin_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, Fp, rows, rows + n);
+    gershgorin_bounds_kernel<<<1, 1024, 0, st>>>(n, rows, rows + n, state);
+    sp2_init_kernel<<<tiles, 256, 0, st>>>(n, Fp, state, X);
     const double tol = 1e-13 * std::max(1.0, nocc);
     double *cur = X, *nxt = work;
     int it = 0;
     for (; it < max_iter; ++it) {
-        sp2_step_kernel<<<1, 1024, 0, st>>>(n, nocc, cur, state, it, tol);
-        gemm_f64_ex(n, n, n, false, false, 0.0, state + 4, cur, n, cur, n, 0.0, state + 5, cur, nxt, n, st);
+        sp2_step_kernel<<<tiles, 256, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
         std::swap(cur, nxt);
         if ((it & 7) == 7) {
             double done = 0.0;
-            PF_HIP(hipMemcpyAsync(&done, state + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+            PF_HIP(hipMemcpyAsync(&done, state + 3 + 2 * ((it + 1) % 3), sizeof(double), hipMemcpyDeviceToHost, st));
             PF_HIP(hipStreamSynchronize(st));
             if (done != 0.0) break;
         }
     }
     // symmetrise (X + X^T)/2 into X
+    const dim3 g2((n + 127) / 128, n);
     if (cur == X) {
         symmetrize_kernel<<<g2, 128, 0, st>>>(n, X, work);
         PF_HIP(hipMemcpyAsync(X, work, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
