@@ -78,6 +78,49 @@ def cpu_baseline(species, xyz, n_builds, target_s):
     }
 
 
+def batched(pyfacc, count):
+    """The batched workload: `count` seeded clusters of 1-12 waters (about 10 %
+    with an OH radical, UHF), STO-3G, each to |dE| < 1e-8 and ||dD|| < 1e-8,
+    in one pf_batch_scf call per precision form (setup, stored integrals and
+    all SCF iterations are inside the timed call)."""
+    from pyfacc import clusters
+    mols = []
+    for i in range(count):
+        sp, xyz, _, _ = clusters.batched_instance(i)
+        ne = sum(pyfacc.helper_functions.periodic_table[s] for s in sp)
+        mols.append((sp, np.asarray(xyz) * ANG2BOHR, (ne + 1) // 2, ne // 2))
+    opt = dict(eps=1e-8, max_iter=100, diis=8, schwarz=1e-10)
+    res = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        t0 = time.perf_counter()
+        r = pyfacc.batch_scf(mols, graph=True, precision=prec, **opt)
+        res[name] = (time.perf_counter() - t0, r)
+    (t64, r64), (tmx, rmx) = res["fp64"], res["mixed"]
+    st64, stmx = r64["stats"], rmx["stats"]
+    return {
+        "instances": count,
+        "uhf_instances": int(sum(1 for m in mols if m[2] != m[3])),
+        "nbf_mean": float(np.mean([5 * m[0].count("O") + m[0].count("H") for m in mols])),  # STO-3G
+        "scf_per_s_mixed": count / tmx,
+        "scf_per_s_fp64": count / t64,
+        "time_mixed_s": tmx,
+        "time_fp64_s": t64,
+        "scf_loop_mixed_s": stmx["t_scf"],
+        "scf_loop_fp64_s": st64["t_scf"],
+        "setup_s": st64["t_setup"],
+        "eri_build_s": st64["t_eri"],
+        "stored_integrals": st64["eri_values"],
+        "store_gb": st64["store_bytes"] / 1e9,
+        "iterations_fp64": int(st64["iterations"]),
+        "iterations_mixed": int(stmx["iterations"]),
+        "converged_fp64": int(r64["converged"].sum()),
+        "converged_mixed": int(rmx["converged"].sum()),
+        "max_abs_dE_mixed_vs_fp64": float(np.abs(rmx["energy"] - r64["energy"]).max()),
+        "hipgraph": bool(st64["graph"]),
+        "digest_waves_autotuned": int(st64["digest_waves"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +129,8 @@ def main():
     ap.add_argument("--xyz", default=os.path.join(ROOT, "pyfacc-scf_amd", "data", "molecules", "h2o_64.xyz"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batch", type=int, default=4096,
+                    help="instances of the batched water-cluster workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -114,13 +159,13 @@ def main():
     t0 = time.perf_counter()
     mixed = [pyfacc.scf(species, xyz, nocc, nocc, options=optmx) for _ in range(args.steps)]
     t_mixed = time.perf_counter() - t0
-    q_mx, ms_mx, n_mx, orient = pyfacc.scf_profile()
+    q_mx, ms_mx, n_mx, orient, p_mx = pyfacc.scf_profile(with_prims=True)
 
     # ---- comparison: plain FP64 and FP64-incremental, timed the same way
     t0 = time.perf_counter()
     fp64 = [pyfacc.scf(species, xyz, nocc, nocc, options=opt64) for _ in range(args.steps)]
     t_fp64 = time.perf_counter() - t0
-    q_64, ms_64, n_64, orient64 = pyfacc.scf_profile()
+    q_64, ms_64, n_64, orient64, p_64 = pyfacc.scf_profile(with_prims=True)
     t0 = time.perf_counter()
     inc = pyfacc.scf(species, xyz, nocc, nocc, options=optinc)
     t_inc = time.perf_counter() - t0
@@ -134,20 +179,24 @@ def main():
     launches = int(n_mx[prec, cls])
     avg_ms = flat_ms[k] / max(launches, 1)
     q_per_launch = q_mx[prec, cls] / max(launches, 1)
-    fl = flops.class_flops(cls, q_per_launch, kprim_pair=9, ndens=1, taylor_terms=7 if prec == 0 else 5,
-                           orient=orient[cls])
+    fl = flops.class_flops(cls, q_per_launch, ndens=1, taylor_terms=7 if prec == 0 else 5, orient=orient[cls],
+                           prim_quartets=p_mx[prec, cls] / max(launches, 1))
     achieved = fl / (avg_ms * 1e-3) / 1e12
     # the same kernel class in the plain FP64 SCF (full builds)
     k64 = int(np.argmax(ms_64[0]))
     avg64 = ms_64[0, k64] / max(n_64[0, k64], 1)
-    fl64 = flops.class_flops(k64, q_64[0, k64] / max(n_64[0, k64], 1), orient=orient64[k64])
-    fock64_flops = flops.fock_flops(q_64[0] / np.maximum(n_64[0], 1), orients=orient64)
+    nb64 = np.maximum(n_64[0], 1)
+    fl64 = flops.class_flops(k64, q_64[0, k64] / nb64[k64], orient=orient64[k64],
+                             prim_quartets=p_64[0, k64] / nb64[k64])
+    fock64_flops = flops.fock_flops(q_64[0] / nb64, orients=orient64, class_prims=p_64[0] / nb64)
     per_class = {flops.CLASS_NAMES[c]: {
         "orient": "".join("sp"[int(x)] for x in orient64[c]),
         "quartets_per_build": int(q_64[0, c] / max(n_64[0, c], 1)),
         "ms_per_build": float(ms_64[0, c] / max(n_64[0, c], 1)),
-        "tflops": float(flops.class_flops(c, q_64[0, c] / max(n_64[0, c], 1), orient=orient64[c]) /
-                        max(ms_64[0, c] / max(n_64[0, c], 1) * 1e-3, 1e-12) / 1e12)} for c in range(6)}
+        "prim_quartets_per_build": int(p_64[0, c] / nb64[c]),
+        "tflops": float(flops.class_flops(c, q_64[0, c] / nb64[c], orient=orient64[c],
+                                          prim_quartets=p_64[0, c] / nb64[c]) /
+                        max(ms_64[0, c] / nb64[c] * 1e-3, 1e-12) / 1e12)} for c in range(6)}
 
     out = {
         "metric": "SCF time-to-converge & Fock builds/s, mixed vs FP64 GPU at |dE|<=1e-6 Eh",
@@ -183,7 +232,7 @@ def main():
         "fp64_build_per_class": per_class,
         "roofline": {
             "bound": "mfma",
-            "kernel": f"jk_kernel {flops.CLASS_NAMES[cls]} {'fp64' if prec == 0 else 'fp32'}",
+            "kernel": f"jk_direct_kernel {flops.CLASS_NAMES[cls]} {'fp64' if prec == 0 else 'fp32'}",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS if prec == 0 else 157.3,
             "unit": "TFLOP/s",
@@ -196,6 +245,8 @@ def main():
                                          "tflops": fl64 / (avg64 * 1e-3) / 1e12},
         },
     }
+    if args.batch > 0:
+        out["batched"] = batched(pyfacc, args.batch)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(species, xyz, fp64[-1]["fock_builds_fp64"], args.cpu_seconds)
     print(json.dumps(out))

@@ -145,10 +145,43 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
  * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,
  * 1 FP32; classes (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)):
  * quartets that passed screening, kernel milliseconds (HIP events on the
- * engine's stream) and kernel launches; class_l[24] gives per class the
- * angular momenta (la lb | lc ld) in the orientation launched.  Any output
- * may be NULL. */
-int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l);
+ * engine's stream), kernel launches and primitive quartets evaluated (the
+ * significant primitive pairs of bra times ket, summed over quartets);
+ * class_l[24] gives per class the angular momenta (la lb | lc ld) in the
+ * orientation launched.  Any output may be NULL. */
+int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l,
+                   unsigned long long *prims);
+
+/* Statistics of one pf_batch_scf call. */
+typedef struct {
+    double t_total;     /* seconds, whole call */
+    double t_setup;     /* integrals, pairs, stored ERIs and orthogonalisers */
+    double t_eri;       /* of which: computing the stored integrals */
+    double t_scf;       /* the SCF iterations */
+    double eri_values;  /* unique integrals stored (each kept in FP64 and FP32) */
+    double store_bytes; /* device bytes of the stored integrals */
+    int iterations;     /* batched iterations run (the slowest molecule's count) */
+    int graph;          /* 1: iterations replayed from one captured hipGraph */
+    int digest_waves;   /* autotuned wavefronts per molecule in the J/K digest */
+    int nproblems;      /* spin densities in the batch (1 per RHF, 2 per UHF molecule) */
+} pf_batch_stats;
+
+/* Batched SCF over many independent molecules in one set of launches.  No
+ * counterpart in the reference: it is for screening workloads (thousands of
+ * small water clusters) where one molecule cannot fill the GPU.  Each
+ * molecule follows the reference's RHF/UHF iteration and stopping rule
+ * (rscf.f90:49-164, uscf.f90:65-213; UHF when nalpha != nbeta, core guess),
+ * with DIIS and integrals computed once and stored.  Molecule m has natoms[m]
+ * atoms; species (2 chars per atom, as pf_scf), z and coords (bohr, xyz per
+ * atom) are concatenated in molecule order.  opt: eps, max_iter, diis (2..8;
+ * 0 or 1 = plain Roothaan), schwarz, renormalize, precision (FP64, or mixed:
+ * FP32 stored integrals until ||dD||_F < mixed_switch, then FP64 to
+ * convergence).  use_graph: replay the iteration from a captured hipGraph.
+ * Outputs per molecule (may be NULL): total energy (Eh), iterations,
+ * converged flag. */
+int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int32_t *z, const double *coords,
+                 const char *basis_name, const int32_t *nalpha, const int32_t *nbeta, const pf_options *opt,
+                 int use_graph, double *energy, int32_t *iterations, int32_t *converged, pf_batch_stats *stats);
 
 /* Device kernels exposed for testing on host buffers. */
 int pf_boys(int nmax, int npts, const double *x, double *F);               /* F[npts*(nmax+1)], nmax <= 6 */

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "engine.h"
 #include "linalg.h"
 
@@ -16,6 +17,7 @@ double device_copy(size_t bytes, hipStream_t st);
 namespace {
 thread_local std::string g_last_error;
 thread_local std::array<unsigned long long, 12> g_prof_q{};
+thread_local std::array<unsigned long long, 12> g_prof_p{};
 thread_local std::array<double, 12> g_prof_ms{};
 thread_local std::array<int, 12> g_prof_n{};
 thread_local std::array<int, 24> g_prof_l{};
@@ -167,10 +169,45 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
         pf_result r;
         eng.scf(nalpha, nbeta, o, r, D_out, eps_out);
         g_prof_q = eng.prof_quartets;
+        g_prof_p = eng.prof_prims;
         g_prof_ms = eng.prof_ms;
         g_prof_n = eng.prof_launches;
         g_prof_l = eng.prof_class_l;
         if (res) *res = r;
+    });
+}
+
+int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int32_t *z, const double *coords,
+                 const char *basis_name, const int32_t *nalpha, const int32_t *nbeta, const pf_options *opt,
+                 int use_graph, double *energy, int32_t *iterations, int32_t *converged, pf_batch_stats *stats) {
+    return guarded([&] {
+        pf_options o;
+        if (opt) o = *opt;
+        else pf_default_options(&o);
+        if (o.max_iter < 1) throw Error("max_iter must be >= 1");
+        if (nmol <= 0) throw Error("nmol must be positive");
+        if (!natoms || !nalpha || !nbeta) throw Error("null natoms, nalpha or nbeta");
+        std::vector<BatchMolecule> mols(nmol);
+        size_t a0 = 0;
+        for (int m = 0; m < nmol; ++m) {
+            if (natoms[m] <= 0) throw Error("batch: molecule " + std::to_string(m) + " has no atoms");
+            Input in = make_input(natoms[m], species + 2 * a0, z + a0, coords + 3 * a0);
+            mols[m].mol = in.mol;
+            mols[m].species = in.species;
+            mols[m].nalpha = nalpha[m];
+            mols[m].nbeta = nbeta[m];
+            a0 += natoms[m];
+        }
+        BatchEngine be(std::move(mols), resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,
+                       o.schwarz > 0 ? o.schwarz : 1e-14);
+        BatchStats bs;
+        be.run(o, use_graph != 0, energy, iterations, converged, &bs);
+        if (stats) {
+            stats->t_total = bs.t_total; stats->t_setup = bs.t_setup; stats->t_eri = bs.t_eri;
+            stats->t_scf = bs.t_scf; stats->eri_values = bs.eri_values; stats->store_bytes = bs.store_bytes;
+            stats->iterations = bs.iterations; stats->graph = bs.graph; stats->digest_waves = bs.digest_waves;
+            stats->nproblems = bs.nproblems;
+        }
     });
 }
 
@@ -193,9 +230,11 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
     });
 }
 
-int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l) {
+int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l,
+                   unsigned long long *prims) {
     for (int q = 0; q < 12; ++q) {
         if (quartets) quartets[q] = g_prof_q[q];
+        if (prims) prims[q] = g_prof_p[q];
         if (ms) ms[q] = g_prof_ms[q];
         if (launches) launches[q] = g_prof_n[q];
     }

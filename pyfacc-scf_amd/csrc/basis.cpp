@@ -92,13 +92,21 @@ std::string resolve_basis_path(const std::string &name) {
     throw Error(msg);
 }
 
-Basis build_basis(const std::string &path, const std::vector<std::string> &species, const Molecule &mol,
-                  bool renormalise) {
+std::vector<std::string> read_basis_file(const std::string &path) {
     std::ifstream in(path);
     if (!in) throw Error("cannot open basis file " + path);
     std::vector<std::string> lines;
     for (std::string l; std::getline(in, l);) lines.push_back(l);
+    return lines;
+}
 
+Basis build_basis(const std::string &path, const std::vector<std::string> &species, const Molecule &mol,
+                  bool renormalise) {
+    return build_basis_lines(read_basis_file(path), path, species, mol, renormalise);
+}
+
+Basis build_basis_lines(const std::vector<std::string> &lines, const std::string &path,
+                        const std::vector<std::string> &species, const Molecule &mol, bool renormalise) {
     Basis basis;
     size_t i = 0;
     while (i < lines.size()) {

@@ -1,0 +1,115 @@
+// Batched SCF over many independent small molecules on one GPU.
+//
+// Every molecule's unique shell quartets (Schwarz-screened) are computed once
+// into stored FP64 and FP32 integral lists; each SCF iteration then runs a
+// fixed sequence of batched kernels over all molecules (spin densities are
+// "problems": one per RHF molecule, two per UHF one):
+//   digest (one workgroup per molecule, J/K in LDS) -> F = H + G,
+//   DIIS (error FDS - SDF, B matrix, per-molecule solve, extrapolation),
+//   F' = X^T F X, SP2 purification, D = occ X P X^T, energy and convergence.
+// Converged molecules are frozen.  The sequence contains no host decision, so
+// it is captured once into a hipGraph and replayed every iteration; the host
+// reads one counter (molecules still running) per iteration.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+#include "pf_common.h"
+
+namespace pf {
+
+struct BatchMolecule {
+    Molecule mol;
+    std::vector<std::string> species;
+    int nalpha = 0, nbeta = 0;
+};
+
+// Device view of the stored integrals of one quartet class for the digest.
+struct BatchClass {
+    int la, lb, lc, ld;
+    const int *sa = nullptr, *sb = nullptr, *oa = nullptr, *ob = nullptr;  // bra pairs
+    const int *ksa = nullptr, *ksb = nullptr, *koa = nullptr, *kob = nullptr;  // ket pairs
+    const long long *off = nullptr;  // per bra: first stored value
+    const int *nket = nullptr, *kbase = nullptr;
+    const double *v64 = nullptr;
+    const float *v32 = nullptr;
+    const int *brange = nullptr;  // per molecule: bra pairs brange[2m] .. brange[2m+1]-1
+    int same = 0;
+};
+
+// Per-molecule and per-problem state shared by the batched kernels.
+struct BatchDev {
+    int M = 0, P = 0, nmax = 0, ndiis = 0;
+    const int *mn = nullptr, *muhf = nullptr, *mprob = nullptr;  // nbf, UHF flag, first problem
+    const long long *moff = nullptr;                              // molecule matrix offsets
+    const int *pn = nullptr, *pmol = nullptr;
+    const double *poccf = nullptr, *pnocc = nullptr;              // occupation factor, occupied count
+    const long long *poff = nullptr, *pmoff = nullptr;            // problem / its molecule's matrix offsets
+    const long long *pdiis = nullptr, *psp2 = nullptr;            // DIIS ring and SP2 state offsets
+    int *mactive = nullptr;   // 1 while the molecule iterates
+    int *mprec = nullptr;     // mixed form: 0 = FP32 integrals, 1 = FP64
+    int *mhead = nullptr, *mnstore = nullptr, *muse = nullptr;  // DIIS ring head / size / extrapolate flag
+    double *mcoef = nullptr;  // DIIS coefficients [M][ndiis]
+    double *mB = nullptr;     // DIIS B matrices [M][ndiis][ndiis]
+};
+
+struct BatchStats {
+    double t_total = 0, t_setup = 0, t_eri = 0, t_scf = 0;
+    double eri_values = 0, store_bytes = 0;
+    int iterations = 0, graph = 0, digest_waves = 0, nproblems = 0;
+};
+
+class BatchEngine {
+   public:
+    BatchEngine(std::vector<BatchMolecule> mols, const std::string &basis_path, bool renormalise, double schwarz);
+    ~BatchEngine();
+    BatchEngine(const BatchEngine &) = delete;
+    BatchEngine &operator=(const BatchEngine &) = delete;
+    // energy/iterations/converged per molecule; use_graph: replay iterations from a hipGraph
+    void run(const pf_options &opt, bool use_graph, double *energy, int *iterations, int *converged,
+             BatchStats *stats);
+
+   private:
+    void setup();
+    void setup_orthogonalizer();
+    void density_step(const double *F);  // D from F (per problem) for the active molecules
+    void iteration(const pf_options &opt);
+    void digest(int waves);
+
+    hipStream_t st_ = nullptr;
+    std::vector<BatchMolecule> mols_;
+    std::string basis_path_;
+    bool renorm_;
+    double schwarz_;
+    double t_eri_ = 0.0;
+    int M_ = 0, P_ = 0, nmax_ = 0, ndiis_ = 8;
+    std::vector<int> mn_, muhf_, mprob_, pn_, pmol_;
+    std::vector<double> enuc_;
+    std::vector<long long> moff_, poff_;
+    long long mtot_ = 0, ptot_ = 0;
+    std::vector<Shell> shells_;
+    // device
+    DevBuf<double> tab_;
+    DevBuf<int> d_mn_, d_muhf_, d_mprob_, d_pn_, d_pmol_, d_mactive_, d_mprec_, d_mhead_, d_mnstore_, d_muse_;
+    DevBuf<int> d_count_;
+    DevBuf<double> d_poccf_, d_pnocc_, d_enuc_, d_E_, d_Eold_, d_iters_, d_conv_, d_coef_, d_B_;
+    DevBuf<long long> d_moff_, d_poff_, d_pmoff_, d_pdiis_, d_psp2_;
+    DevBuf<double> S_, H_, X_, Dtot_;                    // molecule pools
+    DevBuf<double> D_, Dold_, F_, Fx_, Fp_, W1_, W2_, Pr_, sp2_;  // problem pools
+    DevBuf<double> dF_, dE_;                             // DIIS rings
+    std::array<PairSet, 3> pairs_;
+    std::array<std::vector<int>, 3> pbeg_;  // per class: molecule m's pairs pbeg[m] .. pbeg[m+1]-1
+    // stored integrals per quartet class
+    std::array<DevBuf<double>, 6> s64_;
+    std::array<DevBuf<float>, 6> s32_;
+    std::array<DevBuf<long long>, 6> soff_;
+    std::array<DevBuf<int>, 6> snket_, skbase_, sbr_;
+    std::array<BatchClass, 6> cls_;
+    BatchDev dev_;
+    double eri_values_ = 0;
+    int tuned_waves_ = 0;
+};
+
+}  // namespace pf

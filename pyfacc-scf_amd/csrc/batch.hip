@@ -1,0 +1,840 @@
+// Batched SCF over many independent molecules: host driver and the batched
+// dense kernels (per-problem MFMA GEMM tiles, DIIS, SP2, energy/convergence).
+// Batch-level layout: molecule matrices (S, H, X, D_alpha + D_beta) and
+// problem matrices (D, F, DIIS rings, SP2 iterates) each live in one pool,
+// matrix of molecule/problem k at offset off[k] (n_k^2 doubles, row-major).
+// Kernels take the problem (or molecule) from blockIdx.x and skip frozen
+// (converged) molecules.
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+
+#include "batch.h"
+#include "bdense.h"
+#include "boys.h"
+#include "cart.h"
+#include "linalg.h"
+
+namespace pf {
+
+size_t batch_digest_lds(int nmax);
+void batch_digest_prepare(int nmax);
+void launch_batch_digest(const BatchDev &b, const std::array<BatchClass, 6> &cls, const double *D,
+                         const double *Dtot, const double *H, double *F, int waves, hipStream_t st);
+
+namespace {
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+__host__ __device__ inline int ntile(int n) { return ((n + 31) / 32) * ((n + 31) / 32); }
+long long round64(long long n) { return (n + 63) / 64 * 64; }
+size_t sp2_size(int n) { return 8 + 6 * (size_t)ntile(n) + 2 * (size_t)n; }
+constexpr int NSP2 = 64;  // purification steps per density (even: the result ends in the first buffer)
+
+// C_p = alpha a_p op(A_p) op(B_p) + beta Cin_p for each problem p (a_p = ascale[p], or 1).
+struct BGemm {
+    const double *A, *B, *Cin;
+    double *C;
+    const long long *oA, *oB, *oCin, *oC;
+    const double *ascale;
+    int ta, tb;
+    double alpha, beta;
+};
+__global__ void __launch_bounds__(256) bgemm_kernel(BatchDev b, BGemm g) {
+    __shared__ double red[4][16][64];
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]]) return;
+    const int nt = (n + 31) / 32, t = blockIdx.y;
+    if (t >= nt * nt) return;
+    const int r0 = (t / nt) * 32, c0 = (t % nt) * 32;
+    mfma_tile(n, n, n, g.ta, g.tb, g.A + g.oA[p], n, g.B + g.oB[p], n, r0, c0, red);
+    const double al = g.ascale ? g.alpha * g.ascale[p] : g.alpha;
+    double *C = g.C + g.oC[p];
+    const double *Cin = g.Cin ? g.Cin + g.oCin[p] : nullptr;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
+        int gr, gc;
+        tile_pos(idx, r0, c0, gr, gc);
+        if (gr < n && gc < n) {
+            const double v = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+            const size_t o = (size_t)gr * n + gc;
+            C[o] = Cin ? al * v + g.beta * Cin[o] : al * v;
+        }
+    }
+}
+
+// grid (P, chunks): element loop over problem p's n^2 entries
+#define PROBLEM_LOOP(b)                                   \
+    const int p = blockIdx.x, n = b.pn[p];                \
+    if (!b.mactive[b.pmol[p]]) return;                    \
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < n * n; e += gridDim.y * 256)
+
+// dst_p = src_p (src offsets per problem: os)
+__global__ void __launch_bounds__(256) bcopy_kernel(BatchDev b, const double *src, const long long *os, double *dst) {
+    PROBLEM_LOOP(b) dst[b.poff[p] + e] = src[os[p] + e];
+}
+// dst_p = (A_p + A_p^T) / 2
+__global__ void __launch_bounds__(256) bsym_kernel(BatchDev b, const double *A, double *dst) {
+    PROBLEM_LOOP(b) {
+        const int r = e / n, c = e - r * n;
+        dst[b.poff[p] + e] = 0.5 * (A[b.poff[p] + e] + A[b.poff[p] + (size_t)c * n + r]);
+    }
+}
+// DIIS insert at the molecule's ring head: E = W - W^T (W = F D S), and F
+__global__ void __launch_bounds__(256) bdiis_insert_kernel(BatchDev b, const double *W, const double *F, double *dE,
+                                                           double *dF) {
+    PROBLEM_LOOP(b) {
+        const int r = e / n, c = e - r * n;
+        const size_t slot = (size_t)b.mhead[b.pmol[p]] * n * n;
+        dE[b.pdiis[p] + slot + e] = W[b.poff[p] + e] - W[b.poff[p] + (size_t)c * n + r];
+        dF[b.pdiis[p] + slot + e] = F[b.poff[p] + e];
+    }
+}
+// B[m][slot][k] = sum over the molecule's spins of E_slot . E_k, k < new ring size
+__global__ void __launch_bounds__(256) bdiis_dots_kernel(BatchDev b, const double *dE) {
+    __shared__ double sh[4];
+    const int m = blockIdx.x, k = blockIdx.y;
+    if (!b.mactive[m]) return;
+    const int slot = b.mhead[m], ns = min(b.mnstore[m] + 1, b.ndiis);
+    if (k >= ns) return;
+    double s = 0.0;
+    const int nd = b.muhf[m] ? 2 : 1;
+    for (int sp = 0; sp < nd; ++sp) {
+        const int p = b.mprob[m] + sp, n = b.pn[p];
+        const double *es = dE + b.pdiis[p] + (size_t)slot * n * n, *ek = dE + b.pdiis[p] + (size_t)k * n * n;
+        for (int e = threadIdx.x; e < n * n; e += 256) s += es[e] * ek[e];
+    }
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) {
+        b.mB[((size_t)m * b.ndiis + slot) * b.ndiis + k] = s;
+        b.mB[((size_t)m * b.ndiis + k) * b.ndiis + slot] = s;
+    }
+}
+// One thread per molecule: solve [B -1; -1 0][c; l] = [0; -1] over the ring
+// (Gaussian elimination with partial pivoting), advance the ring.
+__global__ void bdiis_solve_kernel(BatchDev b) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= b.M || !b.mactive[m]) return;
+    constexpr int NMAX = 9;  // ring size <= 8
+    const int nd = b.ndiis, ns = min(b.mnstore[m] + 1, nd), N = ns + 1;
+    double A[NMAX][NMAX + 1];
+    int ok = ns >= 2;
+    if (ok) {
+        for (int r = 0; r < ns; ++r) {
+            for (int c = 0; c < ns; ++c) A[r][c] = b.mB[((size_t)m * nd + r) * nd + c];
+            A[r][ns] = -1.0;
+            A[r][N] = 0.0;
+        }
+        for (int c = 0; c < ns; ++c) A[ns][c] = -1.0;
+        A[ns][ns] = 0.0;
+        A[ns][N] = -1.0;
+        for (int col = 0; col < N && ok; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < N; ++r)
+                if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+            if (fabs(A[piv][col]) < 1e-300) { ok = 0; break; }
+            if (piv != col)
+                for (int k = 0; k <= N; ++k) { const double t = A[col][k]; A[col][k] = A[piv][k]; A[piv][k] = t; }
+            for (int r = 0; r < N; ++r) {
+                if (r == col) continue;
+                const double f = A[r][col] / A[col][col];
+                for (int k = col; k <= N; ++k) A[r][k] -= f * A[col][k];
+            }
+        }
+        if (ok)
+            for (int r = 0; r < ns; ++r) b.mcoef[(size_t)m * nd + r] = A[r][N] / A[r][r];
+    }
+    b.muse[m] = ok;
+    b.mhead[m] = (b.mhead[m] + 1) % nd;
+    b.mnstore[m] = ns;
+}
+// Fx_p = sum_k c_k F_k (DIIS) or F_p
+__global__ void __launch_bounds__(256) bdiis_extrap_kernel(BatchDev b, const double *F, const double *dF,
+                                                           double *Fx) {
+    PROBLEM_LOOP(b) {
+        const int m = b.pmol[p];
+        double v;
+        if (b.muse[m]) {
+            v = 0.0;
+            for (int k = 0; k < b.mnstore[m]; ++k)
+                v += b.mcoef[(size_t)m * b.ndiis + k] * dF[b.pdiis[p] + (size_t)k * n * n + e];
+        } else {
+            v = F[b.poff[p] + e];
+        }
+        Fx[b.poff[p] + e] = v;
+    }
+}
+// D_alpha + D_beta of UHF molecules
+__global__ void __launch_bounds__(256) bdtot_kernel(BatchDev b, const double *D, double *Dtot) {
+    PROBLEM_LOOP(b) {
+        const int m = b.pmol[p];
+        if (!b.muhf[m] || p != b.mprob[m]) return;
+        Dtot[b.moff[m] + e] = D[b.poff[p] + e] + D[b.poff[p + 1] + e];
+    }
+}
+
+// ---- SP2 per problem (state layout as sp2_density in linalg.hip)
+__global__ void __launch_bounds__(256) bgersh_rows_kernel(BatchDev b, const double *A, double *st) {
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]]) return;
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const double *Ap = A + b.poff[p];
+    double r = 0.0;
+    for (int j = lane; j < n; j += 64)
+        if (j != i) r += fabs(Ap[(size_t)i * n + j]);
+    for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
+    if (lane == 0) {
+        double *rows = st + b.psp2[p] + 8 + 6 * (size_t)ntile(n);
+        const double d = Ap[(size_t)i * n + i];
+        rows[i] = d - r;
+        rows[n + i] = d + r;
+    }
+}
+__global__ void __launch_bounds__(256) bgersh_bounds_kernel(BatchDev b, double *st) {
+    __shared__ double lo_s[4], hi_s[4];
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]]) return;
+    double *s = st + b.psp2[p];
+    const double *rows = s + 8 + 6 * (size_t)ntile(n);
+    double lo = 1e300, hi = -1e300;
+    for (int i = threadIdx.x; i < n; i += 256) { lo = fmin(lo, rows[i]); hi = fmax(hi, rows[n + i]); }
+    for (int o = 32; o >= 1; o >>= 1) { lo = fmin(lo, __shfl_xor(lo, o, 64)); hi = fmax(hi, __shfl_xor(hi, o, 64)); }
+    if ((threadIdx.x & 63) == 0) { lo_s[threadIdx.x >> 6] = lo; hi_s[threadIdx.x >> 6] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < 4; ++w) { lo = fmin(lo, lo_s[w]); hi = fmax(hi, hi_s[w]); }
+        s[0] = lo;
+        s[1] = hi;
+    }
+}
+__global__ void __launch_bounds__(256) bsp2_init_kernel(BatchDev b, const double *Fp, double *st, double *X) {
+    __shared__ double sh[4];
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]]) return;
+    const int nt = (n + 31) / 32, tile = blockIdx.y, nti = nt * nt;
+    if (tile >= nti) return;
+    double *s = st + b.psp2[p];
+    const double *F = Fp + b.poff[p];
+    double *Xp = X + b.poff[p];
+    const int r0 = (tile / nt) * 32, c0 = (tile % nt) * 32;
+    const double lo = s[0], hi = s[1], sc = 1.0 / (hi - lo);
+    double t1 = 0.0, t2 = 0.0;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int r = r0 + (e >> 5), c = c0 + (e & 31);
+        if (r < n && c < n) {
+            const double v = ((r == c ? hi : 0.0) - F[(size_t)r * n + c]) * sc;
+            Xp[(size_t)r * n + c] = v;
+            t2 += v * v;
+            if (r == c) t1 += v;
+        }
+    }
+    t1 = block_sum256(t1, sh);
+    t2 = block_sum256(t2, sh);
+    if (threadIdx.x == 0) {
+        s[8 + tile] = t1;
+        s[8 + nti + tile] = t2;
+        if (tile == 0) { s[2] = 1e300; s[3] = 0.0; }
+    }
+}
+__global__ void __launch_bounds__(256) bsp2_step_kernel(BatchDev b, int k, const double *X, double *Xo, double *st) {
+    __shared__ double red[4][16][64];
+    __shared__ double sh[4];
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]]) return;
+    const int nt = (n + 31) / 32, tile = blockIdx.y, nti = nt * nt;
+    if (tile >= nti) return;
+    double *s = st + b.psp2[p];
+    const int s0 = k % 3, s1 = (k + 1) % 3;
+    const double age = s[3 + 2 * s0];
+    if (age >= 2.0) {
+        if (tile == 0 && threadIdx.x == 0) { s[2 + 2 * s1] = s[2 + 2 * s0]; s[3 + 2 * s1] = age + 1.0; }
+        return;
+    }
+    const double *P = s + 8;
+    double t1 = 0.0, t2 = 0.0;
+    for (int q = threadIdx.x; q < nti; q += 256) { t1 += P[(2 * s0) * nti + q]; t2 += P[(2 * s0 + 1) * nti + q]; }
+    t1 = block_sum256(t1, sh);
+    t2 = block_sum256(t2, sh);
+    const double nocc = b.pnocc[p], tol = 1e-13 * fmax(1.0, nocc);
+    const double err = fabs(t1 - t2), prev = s[2 + 2 * s0];
+    bool done = age != 0.0;
+    if (!done && k > 2 && (err < tol || (err < 1e-8 && err >= prev))) done = true;
+    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
+    const double *Xp = X + b.poff[p];
+    double *Xq = Xo + b.poff[p];
+    const int r0 = (tile / nt) * 32, c0 = (tile % nt) * 32;
+    if (!done) mfma_tile(n, n, n, 0, 0, Xp, n, Xp, n, r0, c0, red);
+    double u1 = 0.0, u2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
+        int gr, gc;
+        tile_pos(idx, r0, c0, gr, gc);
+        if (gr < n && gc < n) {
+            const size_t o = (size_t)gr * n + gc;
+            double v;
+            if (done) {
+                v = Xp[o];
+            } else {
+                const double x2 = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+                v = square ? x2 : 2.0 * Xp[o] - x2;
+            }
+            Xq[o] = v;
+            u2 += v * v;
+            if (gr == gc) u1 += v;
+        }
+    }
+    u1 = block_sum256(u1, sh);
+    u2 = block_sum256(u2, sh);
+    if (threadIdx.x == 0) {
+        s[8 + (2 * s1) * nti + tile] = u1;
+        s[8 + (2 * s1 + 1) * nti + tile] = u2;
+        if (tile == 0) { s[2 + 2 * s1] = err; s[3 + 2 * s1] = done ? age + 1.0 : 0.0; }
+    }
+}
+
+// Energy, convergence and freezing, one workgroup per molecule (rscf.f90:142-156
+// stopping rule: |dE| < eps and ||dD||_F < eps).  Mixed form: integrals switch
+// from FP32 to FP64 when ||dD||_F < mixed_switch (DIIS restarts); only the FP64
+// phase may converge.
+struct ConvArgs {
+    const double *D, *Dold, *F, *H, *enuc;
+    double *E, *Eold, *iters, *conv;
+    int *count;
+    double eps, mixed_switch;
+    int max_iter;
+};
+__global__ void __launch_bounds__(256) benergy_kernel(BatchDev b, ConvArgs a) {
+    __shared__ double sh[4];
+    const int m = blockIdx.x;
+    if (!b.mactive[m]) return;
+    const int n = b.mn[m], uhf = b.muhf[m], p0 = b.mprob[m];
+    const double *H = a.H + b.moff[m];
+    double e = 0.0, d2 = 0.0;
+    for (int sp = 0; sp <= uhf; ++sp) {
+        const size_t o = b.poff[p0 + sp];
+        for (int i = threadIdx.x; i < n * n; i += 256) {
+            const double d = a.D[o + i], dd = d - a.Dold[o + i];
+            e += uhf ? 0.5 * d * (H[i] + a.F[o + i]) : d * (H[i] + a.F[o + i]);
+            d2 += dd * dd;
+        }
+    }
+    e = block_sum256(e, sh);
+    d2 = block_sum256(d2, sh);
+    if (threadIdx.x == 0) {
+        const double eel = uhf ? e : 0.5 * e;
+        const double dE = fabs(eel - a.Eold[m]), errD = sqrt(d2);
+        const double it = a.iters[m] + 1.0;
+        a.iters[m] = it;
+        a.Eold[m] = eel;
+        a.E[m] = eel + a.enuc[m];
+        bool active = true;
+        if (b.mprec[m] == 0) {
+            if (errD < a.mixed_switch) { b.mprec[m] = 1; b.mnstore[m] = 0; b.mhead[m] = 0; }
+        } else if (dE < a.eps && errD < a.eps) {
+            a.conv[m] = 1.0;
+            active = false;
+        }
+        if (it >= a.max_iter) active = false;
+        if (!active) b.mactive[m] = 0;
+        else atomicAdd(a.count, 1);
+    }
+}
+
+// ---- molecule-level helpers for the orthogonaliser (problems = molecules)
+__global__ void __launch_bounds__(256) ns_init_batch(BatchDev b, const double *S, const double *st, double *Y,
+                                                     double *Z, double *I) {
+    PROBLEM_LOOP(b) {
+        const int r = e / n, c = e - r * n;
+        const size_t o = b.poff[p] + e;
+        Y[o] = S[o] / st[b.psp2[p] + 1];
+        Z[o] = I[o] = r == c ? 1.0 : 0.0;
+    }
+}
+__global__ void __launch_bounds__(256) ns_finish_batch(BatchDev b, const double *Z, const double *st, double *X) {
+    PROBLEM_LOOP(b) X[b.poff[p] + e] = Z[b.poff[p] + e] / sqrt(st[b.psp2[p] + 1]);
+}
+__global__ void __launch_bounds__(256) resid_batch(BatchDev b, const double *T, double *out) {
+    __shared__ double sh[4];
+    const int p = blockIdx.x, n = b.pn[p];
+    double mx = 0.0;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int r = e / n, c = e - r * n;
+        mx = fmax(mx, fabs(T[b.poff[p] + e] - (r == c ? 1.0 : 0.0)));
+    }
+    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) out[p] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+__global__ void fill_int_kernel(int n, int v, int *x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+dim3 pgrid(int P, int nmax) { return dim3(P, std::max(1, std::min(64, (nmax * nmax + 255) / 256))); }
+
+// ---- autotune cache: {"key": value, ...} in a small JSON file
+std::string tune_path() {
+    if (const char *e = std::getenv("PYFACC_TUNE_CACHE")) return e;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&tune_path), &info) && info.dli_fname) {
+        std::string f = info.dli_fname;
+        const size_t s = f.find_last_of('/');
+        return (s == std::string::npos ? std::string(".") : f.substr(0, s)) + "/autotune.json";
+    }
+    return "autotune.json";
+}
+std::map<std::string, int> tune_read() {
+    std::map<std::string, int> m;
+    std::ifstream in(tune_path());
+    if (!in) return m;
+    std::stringstream ss;
+    ss << in.rdbuf();
+    const std::string t = ss.str();
+    size_t i = 0;
+    while ((i = t.find('"', i)) != std::string::npos) {
+        const size_t j = t.find('"', i + 1);
+        if (j == std::string::npos) break;
+        const size_t c = t.find(':', j);
+        if (c == std::string::npos) break;
+        m[t.substr(i + 1, j - i - 1)] = std::atoi(t.c_str() + c + 1);
+        i = t.find_first_of(",}", c);
+        if (i == std::string::npos) break;
+    }
+    return m;
+}
+void tune_write(const std::map<std::string, int> &m) {
+    std::ofstream out(tune_path());
+    if (!out) return;  // read-only location: the choice is simply not cached
+    out << "{";
+    bool first = true;
+    for (auto &kv : m) {
+        out << (first ? "" : ", ") << "\"" << kv.first << "\": " << kv.second;
+        first = false;
+    }
+    out << "}\n";
+}
+
+}  // namespace
+
+BatchEngine::BatchEngine(std::vector<BatchMolecule> mols, const std::string &basis_path, bool renormalise,
+                         double schwarz)
+    : mols_(std::move(mols)), basis_path_(basis_path), renorm_(renormalise), schwarz_(schwarz) {
+    PF_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+}
+
+BatchEngine::~BatchEngine() {
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+void BatchEngine::setup() {
+    M_ = (int)mols_.size();
+    if (M_ == 0) throw Error("batch: no molecules");
+    const auto lines = read_basis_file(basis_path_);
+    std::vector<int> shoff(M_ + 1, 0), aoff(M_ + 1, 0);
+    std::vector<double> axyz;
+    std::vector<int> Z;
+    for (int m = 0; m < M_; ++m) {
+        const auto &bm = mols_[m];
+        Basis bs = build_basis_lines(lines, basis_path_, bm.species, bm.mol, renorm_);
+        for (auto &s : bs.shells) shells_.push_back(s);
+        shoff[m + 1] = (int)shells_.size();
+        aoff[m + 1] = aoff[m] + bm.mol.natoms;
+        axyz.insert(axyz.end(), bm.mol.xyz.begin(), bm.mol.xyz.end());
+        Z.insert(Z.end(), bm.mol.Z.begin(), bm.mol.Z.end());
+        const int n = bs.nbf, uhf = bm.nalpha != bm.nbeta;
+        if (bm.nalpha < 1 || bm.nbeta < 1 || bm.nalpha > n || bm.nbeta > n)
+            throw Error("batch: molecule " + std::to_string(m) +
+                        ": each spin needs between 1 and nbf electrons");
+        mn_.push_back(n);
+        muhf_.push_back(uhf);
+        mprob_.push_back((int)pn_.size());
+        enuc_.push_back(nuclear_repulsion(bm.mol));
+        nmax_ = std::max(nmax_, n);
+        for (int s = 0; s <= uhf; ++s) { pn_.push_back(n); pmol_.push_back(m); }
+    }
+    P_ = (int)pn_.size();
+    moff_.resize(M_);
+    poff_.resize(P_);
+    for (int m = 0; m < M_; ++m) { moff_[m] = mtot_; mtot_ += (long long)mn_[m] * mn_[m]; }
+    std::vector<long long> pmoff(P_), pdiis(P_), psp2(P_);
+    std::vector<double> poccf(P_), pnocc(P_);
+    long long dtot = 0, stot = 0;
+    for (int p = 0; p < P_; ++p) {
+        const int m = pmol_[p], n = pn_[p], spin = p - mprob_[m];
+        poff_[p] = ptot_;
+        ptot_ += (long long)n * n;
+        pmoff[p] = moff_[m];
+        pdiis[p] = dtot;
+        dtot += (long long)ndiis_ * n * n;
+        psp2[p] = stot;
+        stot += (long long)sp2_size(n);
+        poccf[p] = muhf_[m] ? 1.0 : 2.0;
+        pnocc[p] = spin == 0 ? mols_[m].nalpha : mols_[m].nbeta;
+    }
+    // device state
+    const std::vector<double> tab = boys_table_host();
+    tab_.upload(tab.data(), tab.size(), st_);
+    d_mn_.upload(mn_.data(), M_, st_); d_muhf_.upload(muhf_.data(), M_, st_); d_mprob_.upload(mprob_.data(), M_, st_);
+    d_pn_.upload(pn_.data(), P_, st_); d_pmol_.upload(pmol_.data(), P_, st_);
+    d_moff_.upload(moff_.data(), M_, st_); d_poff_.upload(poff_.data(), P_, st_);
+    d_pmoff_.upload(pmoff.data(), P_, st_); d_pdiis_.upload(pdiis.data(), P_, st_); d_psp2_.upload(psp2.data(), P_, st_);
+    d_poccf_.upload(poccf.data(), P_, st_); d_pnocc_.upload(pnocc.data(), P_, st_);
+    d_enuc_.upload(enuc_.data(), M_, st_);
+    d_mactive_.alloc(M_); d_mprec_.alloc(M_); d_mhead_.alloc(M_); d_mnstore_.alloc(M_); d_muse_.alloc(M_);
+    d_count_.alloc(1);
+    d_E_.alloc(M_); d_Eold_.alloc(M_); d_iters_.alloc(M_); d_conv_.alloc(M_);
+    d_coef_.alloc((size_t)M_ * ndiis_); d_B_.alloc((size_t)M_ * ndiis_ * ndiis_);
+    S_.alloc(mtot_); H_.alloc(mtot_); X_.alloc(mtot_); Dtot_.alloc(mtot_);
+    for (auto *b : {&D_, &Dold_, &F_, &Fx_, &Fp_, &W1_, &W2_, &Pr_}) b->alloc(ptot_);
+    sp2_.alloc(stot);
+    dF_.alloc(std::max(dtot, 1LL)); dE_.alloc(std::max(dtot, 1LL));
+    dev_.M = M_; dev_.P = P_; dev_.nmax = nmax_; dev_.ndiis = ndiis_;
+    dev_.mn = d_mn_.p; dev_.muhf = d_muhf_.p; dev_.mprob = d_mprob_.p; dev_.moff = d_moff_.p;
+    dev_.pn = d_pn_.p; dev_.pmol = d_pmol_.p; dev_.poccf = d_poccf_.p; dev_.pnocc = d_pnocc_.p;
+    dev_.poff = d_poff_.p; dev_.pmoff = d_pmoff_.p; dev_.pdiis = d_pdiis_.p; dev_.psp2 = d_psp2_.p;
+    dev_.mactive = d_mactive_.p; dev_.mprec = d_mprec_.p; dev_.mhead = d_mhead_.p; dev_.mnstore = d_mnstore_.p;
+    dev_.muse = d_muse_.p; dev_.mcoef = d_coef_.p; dev_.mB = d_B_.p;
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 1, d_mactive_.p);
+
+    // one-electron integrals, one launch per molecule
+    {
+        DevBuf<Shell> dsh;
+        dsh.upload(shells_.data(), shells_.size(), st_);
+        DevBuf<double> dxyz, T(mtot_), V(mtot_);
+        DevBuf<int> dZ;
+        dxyz.upload(axyz.data(), axyz.size(), st_);
+        dZ.upload(Z.data(), Z.size(), st_);
+        for (int m = 0; m < M_; ++m)
+            launch_onee(dsh.p + shoff[m], shoff[m + 1] - shoff[m], mn_[m], mols_[m].mol.natoms, dxyz.p + 3 * aoff[m],
+                        dZ.p + aoff[m], tab_.p, S_.p + moff_[m], T.p + moff_[m], V.p + moff_[m], st_);
+        dev_axpby(mtot_, 1.0, T.p, 1.0, V.p, H_.p, st_);
+        PF_HIP(hipStreamSynchronize(st_));
+    }
+
+    // shell pairs per class, all molecules concatenated; pass 1 gives Schwarz bounds
+    std::array<std::vector<std::pair<int, int>>, 3> all;
+    std::array<std::vector<int>, 3> beg1;
+    for (int c = 0; c < 3; ++c) beg1[c].assign(1, 0);
+    for (int m = 0; m < M_; ++m) {
+        for (int a = shoff[m]; a < shoff[m + 1]; ++a)
+            for (int b = shoff[m]; b <= a; ++b) {
+                int f = a, s = b;
+                if (shells_[b].l > shells_[a].l) std::swap(f, s);
+                all[shells_[f].l + shells_[s].l].push_back({f, s});
+            }
+        for (int c = 0; c < 3; ++c) beg1[c].push_back((int)all[c].size());
+    }
+    const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+    double bound_max = 0.0;
+    std::array<std::vector<double>, 3> Q1;
+    for (int c = 0; c < 3; ++c) {
+        PairSet &ps = pairs_[c];
+        ps.la = cls_l[c][0]; ps.lb = cls_l[c][1];
+        fill_pairset(ps, shells_, all[c], -1.0, bound_max, st_);
+        ps.d_Q.alloc(std::max(ps.n, 1));
+        if (ps.n) launch_schwarz<double>(ps.la, ps.lb, ps.view64(), tab_.p, ps.d_Q.p, st_);
+        Q1[c].resize(ps.n);
+        ps.d_Q.download(Q1[c].data(), ps.n, st_);
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+    // pass 2: per molecule keep Q * Qmax(molecule) >= tau, sorted by Q descending
+    std::array<std::vector<double>, 3> Q;
+    std::array<std::vector<std::pair<int, int>>, 3> kept;
+    for (int c = 0; c < 3; ++c) pbeg_[c].assign(1, 0);
+    for (int m = 0; m < M_; ++m) {
+        double qmax = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (int i = beg1[c][m]; i < beg1[c][m + 1]; ++i) qmax = std::max(qmax, Q1[c][i]);
+        for (int c = 0; c < 3; ++c) {
+            std::vector<int> idx;
+            for (int i = beg1[c][m]; i < beg1[c][m + 1]; ++i)
+                if (Q1[c][i] * qmax >= schwarz_) idx.push_back(i);
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return Q1[c][x] > Q1[c][y]; });
+            for (int i : idx) { kept[c].push_back(all[c][i]); Q[c].push_back(Q1[c][i]); }
+            pbeg_[c].push_back((int)kept[c].size());
+        }
+    }
+    for (int c = 0; c < 3; ++c) fill_pairset(pairs_[c], shells_, kept[c], bound_max, bound_max, st_);
+
+    // quartet plan per class (bra class x >= ket class y) and the integral store
+    PF_HIP(hipStreamSynchronize(st_));
+    const double t0 = now();
+    int q = 0;
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y <= x; ++y, ++q) {
+            const PairSet &B = pairs_[x], &K = pairs_[y];
+            const bool same = x == y;
+            const int nint = nc(B.la) * nc(B.lb) * nc(K.la) * nc(K.lb);
+            std::vector<long long> off(std::max(B.n, 1), 0);
+            std::vector<int> nk(std::max(B.n, 1), 0), kb(std::max(B.n, 1), 0), br(2 * M_);
+            long long tot = 0;
+            for (int m = 0; m < M_; ++m) {
+                br[2 * m] = pbeg_[x][m];
+                br[2 * m + 1] = pbeg_[x][m + 1];
+                const int k0 = pbeg_[y][m], k1 = pbeg_[y][m + 1];
+                for (int i = pbeg_[x][m]; i < pbeg_[x][m + 1]; ++i) {
+                    const double thr = schwarz_ / Q[x][i];
+                    int lo = k0, hi = k1;  // kets sorted by Q descending: prefix with Q >= thr
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (Q[y][mid] >= thr) lo = mid + 1; else hi = mid;
+                    }
+                    int n = lo - k0;
+                    if (same) n = std::min(n, i - k0 + 1);
+                    nk[i] = n;
+                    kb[i] = k0;
+                    off[i] = tot;
+                    tot += (long long)round64(n) * nint;
+                    eri_values_ += (double)n * nint;
+                }
+            }
+            s64_[q].alloc(std::max(tot, 1LL));
+            s32_[q].alloc(std::max(tot, 1LL));
+            soff_[q].upload(off.data(), off.size(), st_);
+            snket_[q].upload(nk.data(), nk.size(), st_);
+            skbase_[q].upload(kb.data(), kb.size(), st_);
+            sbr_[q].upload(br.data(), br.size(), st_);
+            if (B.n) {
+                JKArgs<double> a;
+                a.bra = B.view64(); a.ket = K.view64(); a.same = same; a.tau = schwarz_;
+                a.nbf = nmax_; a.nsh = 0; a.tab = tab_.p;
+                a.store = s64_[q].p; a.store32 = s32_[q].p;
+                a.store_off = soff_[q].p; a.store_nket = snket_[q].p; a.store_kbase = skbase_[q].p;
+                launch_jk_class<double>(B.la, B.lb, K.la, K.lb, 1, MODE_STORE, a, st_);
+            }
+            BatchClass &bc = cls_[q];
+            bc.la = B.la; bc.lb = B.lb; bc.lc = K.la; bc.ld = K.lb;
+            bc.sa = B.d_sa.p; bc.sb = B.d_sb.p; bc.oa = B.d_oa.p; bc.ob = B.d_ob.p;
+            bc.ksa = K.d_sa.p; bc.ksb = K.d_sb.p; bc.koa = K.d_oa.p; bc.kob = K.d_ob.p;
+            bc.off = soff_[q].p; bc.nket = snket_[q].p; bc.kbase = skbase_[q].p;
+            bc.v64 = s64_[q].p; bc.v32 = s32_[q].p; bc.brange = sbr_[q].p; bc.same = same;
+        }
+    PF_HIP(hipStreamSynchronize(st_));
+    t_eri_ = now() - t0;
+    batch_digest_prepare(nmax_);
+    setup_orthogonalizer();
+}
+
+// X = S^{-1/2} per molecule by the coupled Newton-Schulz iteration on batched
+// MFMA GEMMs (as inv_sqrt_spd); a molecule that does not converge falls back
+// to the Jacobi solver's canonical U s^{-1/2}.
+void BatchEngine::setup_orthogonalizer() {
+    // a molecule-level view: problems = molecules
+    std::vector<int> iota(M_);
+    std::iota(iota.begin(), iota.end(), 0);
+    std::vector<long long> st_off(M_);
+    long long stot = 0;
+    for (int m = 0; m < M_; ++m) { st_off[m] = stot; stot += (long long)sp2_size(mn_[m]); }
+    DevBuf<int> dio, ones;
+    DevBuf<long long> dst;
+    dio.upload(iota.data(), M_, st_);
+    dst.upload(st_off.data(), M_, st_);
+    ones.alloc(M_);
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 1, ones.p);
+    BatchDev v = dev_;
+    v.P = M_; v.pn = d_mn_.p; v.pmol = dio.p; v.poff = d_moff_.p; v.pmoff = d_moff_.p; v.psp2 = dst.p;
+    v.mactive = ones.p;
+    DevBuf<double> Y(mtot_), Z(mtot_), T(mtot_), Y2(mtot_), Z2(mtot_), I(mtot_), stb(stot), res(M_);
+    const dim3 g = pgrid(M_, nmax_), gt(M_, ntile(nmax_)), gr(M_, (nmax_ + 3) / 4);
+    bgersh_rows_kernel<<<gr, 256, 0, st_>>>(v, S_.p, stb.p);
+    bgersh_bounds_kernel<<<M_, 256, 0, st_>>>(v, stb.p);
+    ns_init_batch<<<g, 256, 0, st_>>>(v, S_.p, stb.p, Y.p, Z.p, I.p);
+    auto gemm = [&](const double *A, const double *B, double *C, double alpha, double beta, const double *Cin) {
+        BGemm b{A, B, Cin, C, d_moff_.p, d_moff_.p, d_moff_.p, d_moff_.p, nullptr, 0, 0, alpha, beta};
+        bgemm_kernel<<<gt, 256, 0, st_>>>(v, b);
+    };
+    double *y = Y.p, *z = Z.p, *y2 = Y2.p, *z2 = Z2.p;
+    std::vector<double> hr(M_);
+    bool ok = false;
+    for (int round = 0; round < 6 && !ok; ++round) {
+        for (int k = 0; k < 8; ++k) {
+            gemm(z, y, T.p, -0.5, 1.5, I.p);  // (3I - ZY)/2
+            gemm(y, T.p, y2, 1.0, 0.0, nullptr);
+            gemm(T.p, z, z2, 1.0, 0.0, nullptr);
+            std::swap(y, y2);
+            std::swap(z, z2);
+        }
+        gemm(z, y, T.p, 1.0, 0.0, nullptr);
+        resid_batch<<<M_, 256, 0, st_>>>(v, T.p, res.p);
+        res.download(hr.data(), M_, st_);
+        PF_HIP(hipStreamSynchronize(st_));
+        ok = true;
+        for (double r : hr) ok = ok && std::isfinite(r) && r < 1e-11;
+    }
+    ns_finish_batch<<<g, 256, 0, st_>>>(v, z, stb.p, X_.p);
+    PF_HIP(hipStreamSynchronize(st_));
+    for (int m = 0; m < M_; ++m) {
+        if (std::isfinite(hr[m]) && hr[m] < 1e-11) continue;
+        const int n = mn_[m];
+        DevBuf<double> s(n), U((size_t)n * n);
+        jacobi_eigh(n, S_.p + moff_[m], s.p, U.p, st_);
+        std::vector<double> hs(n), hU((size_t)n * n);
+        s.download(hs.data(), n, st_);
+        U.download(hU.data(), hU.size(), st_);
+        PF_HIP(hipStreamSynchronize(st_));
+        if (hs[0] <= 0.0) throw Error("batch: overlap matrix of molecule " + std::to_string(m) + " is not positive definite");
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) hU[(size_t)r * n + c] /= std::sqrt(hs[c]);
+        PF_HIP(hipMemcpyAsync(X_.p + moff_[m], hU.data(), sizeof(double) * hU.size(), hipMemcpyHostToDevice, st_));
+        PF_HIP(hipStreamSynchronize(st_));
+    }
+}
+
+// D_p = occ_p X P X^T with P the projector of F'_p = X^T F_p X (SP2), and D_alpha + D_beta.
+void BatchEngine::density_step(const double *F) {
+    const dim3 g = pgrid(P_, nmax_), gt(P_, ntile(nmax_)), gr(P_, (nmax_ + 3) / 4);
+    auto gemm = [&](const double *A, const long long *oA, int ta, const double *B, const long long *oB, int tb,
+                    double *C, const double *ascale) {
+        BGemm b{A, B, nullptr, C, oA, oB, d_poff_.p, d_poff_.p, ascale, ta, tb, 1.0, 0.0};
+        bgemm_kernel<<<gt, 256, 0, st_>>>(dev_, b);
+    };
+    gemm(F, d_poff_.p, 0, X_.p, d_pmoff_.p, 0, W1_.p, nullptr);      // F X
+    gemm(X_.p, d_pmoff_.p, 1, W1_.p, d_poff_.p, 0, Fp_.p, nullptr);  // X^T F X
+    bgersh_rows_kernel<<<gr, 256, 0, st_>>>(dev_, Fp_.p, sp2_.p);
+    bgersh_bounds_kernel<<<P_, 256, 0, st_>>>(dev_, sp2_.p);
+    bsp2_init_kernel<<<gt, 256, 0, st_>>>(dev_, Fp_.p, sp2_.p, Pr_.p);
+    double *cur = Pr_.p, *nxt = W1_.p;
+    for (int k = 0; k < NSP2; ++k) {
+        bsp2_step_kernel<<<gt, 256, 0, st_>>>(dev_, k, cur, nxt, sp2_.p);
+        std::swap(cur, nxt);
+    }
+    bsym_kernel<<<g, 256, 0, st_>>>(dev_, Pr_.p, W1_.p);                           // NSP2 even: result in Pr
+    gemm(X_.p, d_pmoff_.p, 0, W1_.p, d_poff_.p, 0, W2_.p, nullptr);              // X P
+    gemm(W2_.p, d_poff_.p, 0, X_.p, d_pmoff_.p, 1, D_.p, d_poccf_.p);            // occ X P X^T
+    bdtot_kernel<<<g, 256, 0, st_>>>(dev_, D_.p, Dtot_.p);
+}
+
+void BatchEngine::digest(int waves) { launch_batch_digest(dev_, cls_, D_.p, Dtot_.p, H_.p, F_.p, waves, st_); }
+
+// One SCF iteration for every active molecule (no host decision inside).
+void BatchEngine::iteration(const pf_options &opt) {
+    const dim3 g = pgrid(P_, nmax_), gt(P_, ntile(nmax_));
+    fill_int_kernel<<<1, 64, 0, st_>>>(1, 0, d_count_.p);
+    digest(tuned_waves_);
+    bcopy_kernel<<<g, 256, 0, st_>>>(dev_, D_.p, d_poff_.p, Dold_.p);
+    if (opt.diis > 1) {
+        BGemm a{F_.p, D_.p, nullptr, W1_.p, d_poff_.p, d_poff_.p, d_poff_.p, d_poff_.p, nullptr, 0, 0, 1.0, 0.0};
+        bgemm_kernel<<<gt, 256, 0, st_>>>(dev_, a);  // F D
+        BGemm s{W1_.p, S_.p, nullptr, W2_.p, d_poff_.p, d_pmoff_.p, d_poff_.p, d_poff_.p, nullptr, 0, 0, 1.0, 0.0};
+        bgemm_kernel<<<gt, 256, 0, st_>>>(dev_, s);  // F D S
+        bdiis_insert_kernel<<<g, 256, 0, st_>>>(dev_, W2_.p, F_.p, dE_.p, dF_.p);
+        bdiis_dots_kernel<<<dim3(M_, ndiis_), 256, 0, st_>>>(dev_, dE_.p);
+        bdiis_solve_kernel<<<(M_ + 63) / 64, 64, 0, st_>>>(dev_);
+        bdiis_extrap_kernel<<<g, 256, 0, st_>>>(dev_, F_.p, dF_.p, Fx_.p);
+        density_step(Fx_.p);
+    } else {
+        density_step(F_.p);
+    }
+    ConvArgs c{D_.p, Dold_.p, F_.p, H_.p, d_enuc_.p, d_E_.p, d_Eold_.p, d_iters_.p, d_conv_.p, d_count_.p,
+               opt.eps, opt.mixed_switch, opt.max_iter};
+    benergy_kernel<<<M_, 256, 0, st_>>>(dev_, c);
+}
+
+void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int *iterations, int *converged,
+                      BatchStats *stats) {
+    const double t0 = now();
+    if (opt.diis > 8) throw Error("batch: DIIS subspace of at most 8");
+    ndiis_ = std::max(opt.diis, 2);
+    setup();
+    const double t1 = now();
+    // state: frozen flags, precision, DIIS rings, energies
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, opt.precision == PF_PREC_MIXED ? 0 : 1, d_mprec_.p);
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mhead_.p);
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mnstore_.p);
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_muse_.p);
+    for (auto *b : {&d_E_, &d_Eold_, &d_iters_, &d_conv_}) dev_fill(M_, 0.0, b->p, st_);
+    // core guess: D from H (rscf.f90:113-114; uscf.f90:140-144)
+    bcopy_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, H_.p, d_pmoff_.p, Fx_.p);
+    density_step(Fx_.p);
+    // autotune the digest's wavefronts per molecule (cached per device and batch shape)
+    hipDeviceProp_t prop;
+    int dev = 0;
+    PF_HIP(hipGetDevice(&dev));
+    PF_HIP(hipGetDeviceProperties(&prop, dev));
+    const std::string key = std::string("batch_digest_waves:") + prop.gcnArchName + ":" + std::to_string(M_) + ":" +
+                            std::to_string(nmax_);
+    auto cache = tune_read();
+    if (cache.count(key) && (cache[key] == 4 || cache[key] == 8)) {
+        tuned_waves_ = cache[key];
+    } else {
+        hipEvent_t e0, e1;
+        PF_HIP(hipEventCreate(&e0));
+        PF_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int w : {4, 8}) {
+            digest(w);  // warm
+            PF_HIP(hipEventRecord(e0, st_));
+            digest(w);
+            PF_HIP(hipEventRecord(e1, st_));
+            PF_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; tuned_waves_ = w; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        cache[key] = tuned_waves_;
+        tune_write(cache);
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+    const double t2 = now();
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph) {
+        PF_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        iteration(opt);
+        PF_HIP(hipStreamEndCapture(st_, &graph));
+        PF_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    int it = 0;
+    for (; it < opt.max_iter; ++it) {
+        if (use_graph) PF_HIP(hipGraphLaunch(exec, st_));
+        else iteration(opt);
+        int active = 0;
+        PF_HIP(hipMemcpyAsync(&active, d_count_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+        PF_HIP(hipStreamSynchronize(st_));
+        if (active == 0) { ++it; break; }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    std::vector<double> E(M_), iters(M_), conv(M_);
+    d_E_.download(E.data(), M_, st_);
+    d_iters_.download(iters.data(), M_, st_);
+    d_conv_.download(conv.data(), M_, st_);
+    PF_HIP(hipStreamSynchronize(st_));
+    for (int m = 0; m < M_; ++m) {
+        if (energy) energy[m] = E[m];
+        if (iterations) iterations[m] = (int)iters[m];
+        if (converged) converged[m] = conv[m] != 0.0;
+    }
+    const double t3 = now();
+    if (stats) {
+        stats->t_total = t3 - t0;
+        stats->t_setup = t1 - t0;
+        stats->t_eri = t_eri_;
+        stats->t_scf = t3 - t2;
+        stats->eri_values = eri_values_;
+        double bytes = 0.0;
+        for (int q = 0; q < 6; ++q) bytes += (double)s64_[q].n * 12.0;
+        stats->store_bytes = bytes;
+        stats->iterations = it;
+        stats->graph = use_graph;
+        stats->digest_waves = tuned_waves_;
+        stats->nproblems = P_;
+    }
+}
+
+}  // namespace pf

@@ -27,6 +27,12 @@ struct PairSet {
     PairView<float> view32() const;
 };
 
+// Fill ps from a list of (first, second) shell indices into sh (device SoA,
+// FP64 and FP32 copies; primitive pairs sorted by bound, significant count kpn
+// against kref, kref < 0 keeps all); bound_max is raised to the largest bound.
+void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const std::vector<std::pair<int, int>> &list,
+                  double kref, double &bound_max, hipStream_t st);
+
 struct QuartetClass {
     int bra, ket;  // indices into Engine::pairs_
     int slot;      // unordered class: (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)
@@ -66,6 +72,7 @@ class Engine {
     // (precision 0 = FP64, 1 = FP32): quartets, kernel milliseconds (HIP events
     // on the engine stream, filled when profile_classes is set) and launches.
     std::array<unsigned long long, 12> prof_quartets{};
+    std::array<unsigned long long, 12> prof_prims{};  // primitive quartets evaluated
     std::array<double, 12> prof_ms{};
     std::array<int, 12> prof_launches{};
     std::array<int, 24> prof_class_l{};  // per slot: la lb lc ld as launched (bra | ket)

@@ -98,6 +98,79 @@ void Engine::one_electron(double *S, double *T, double *V) {
     PF_HIP(hipStreamSynchronize(st_));
 }
 
+// Primitive pairs of a shell pair are stored by bound |K|/sqrt(p) descending;
+// kpn counts those with bound * kref >= PRIM_EPS (kref = the largest bound of
+// any pair), the only ones the ERI kernels visit: a primitive quartet is at
+// most K_ab K_cd / sqrt(p+q) times the polynomial factors of the p-type
+// recurrences, so the dropped ones are below 1e-16 in magnitude up to those
+// factors.  kref < 0: keep every primitive (Schwarz pass).  bound_max is
+// raised to the largest bound seen.
+void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const std::vector<std::pair<int, int>> &list,
+                  double kref, double &bound_max, hipStream_t st) {
+    constexpr double PRIM_EPS = 1e-16;
+    ps.n = (int)list.size();
+    ps.npad = round64(std::max(ps.n, 1));
+    int kp = 1;
+    for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
+    ps.kp = kp;
+    std::vector<int> sa(ps.n), sb(ps.n), oa(ps.n), ob(ps.n), kpn(ps.npad, 0);
+    std::vector<double> AB(3 * (size_t)ps.npad, 0.0), prim((size_t)kp * PF_NFIELD * ps.npad, 0.0);
+    for (int u = 0; u < kp; ++u)  // padding entries: unit exponent, zero weight
+        for (int i = 0; i < ps.npad; ++i) {
+            prim[((size_t)u * PF_NFIELD + PF_P) * ps.npad + i] = 1.0;
+            prim[((size_t)u * PF_NFIELD + PF_O2P) * ps.npad + i] = 0.5;
+        }
+    const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
+    std::array<std::array<double, 3>, KMAX * KMAX> order;  // (bound, x, y)
+    for (int i = 0; i < ps.n; ++i) {
+        const Shell &A = sh[list[i].first], &B = sh[list[i].second];
+        sa[i] = list[i].first; sb[i] = list[i].second; oa[i] = A.bf; ob[i] = B.bf;
+        double ab[3], r2 = 0.0;
+        for (int d = 0; d < 3; ++d) { ab[d] = A.xyz[d] - B.xyz[d]; r2 += ab[d] * ab[d]; AB[d * ps.npad + i] = ab[d]; }
+        int no = 0;
+        for (int x = 0; x < A.np; ++x)
+            for (int y = 0; y < B.np; ++y) {
+                const double a = A.exp[x], b = B.exp[y], p = a + b;
+                const double K = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+                order[no++] = {std::fabs(K) / std::sqrt(p), (double)x, (double)y};
+            }
+        std::stable_sort(order.begin(), order.begin() + no, [](const auto &l, const auto &r) { return l[0] > r[0]; });
+        for (int o = 0; o < no; ++o) {
+            bound_max = std::max(bound_max, order[o][0]);
+            if (kref < 0.0 || order[o][0] * kref >= PRIM_EPS) ++kpn[i];
+        }
+        int u = 0;
+        for (int o = 0; o < no; ++o) {
+            const int x = (int)order[o][1], y = (int)order[o][2];
+            const double a = A.exp[x], b = B.exp[y], p = a + b;
+            double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
+            col[PF_P * ps.npad] = p;
+            for (int d = 0; d < 3; ++d) {
+                const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
+                col[(PF_PX + d) * ps.npad] = P;
+                col[(PF_AX + d) * ps.npad] = P - A.xyz[d];
+            }
+            col[PF_K * ps.npad] = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
+            col[PF_O2P * ps.npad] = 0.5 / p;
+            ++u;
+        }
+        for (; u < kp; ++u) {  // padding primitives contribute zero
+            double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
+            col[PF_P * ps.npad] = 1.0;
+            col[PF_K * ps.npad] = 0.0;
+        }
+    }
+    ps.sa = sa; ps.sb = sb;
+    ps.d_sa.upload(sa.data(), ps.n, st); ps.d_sb.upload(sb.data(), ps.n, st);
+    ps.d_oa.upload(oa.data(), ps.n, st); ps.d_ob.upload(ob.data(), ps.n, st);
+    ps.d_kpn.upload(kpn.data(), kpn.size(), st);
+    ps.d_AB.upload(AB.data(), AB.size(), st);
+    ps.d_prim.upload(prim.data(), prim.size(), st);
+    std::vector<float> fAB(AB.begin(), AB.end()), fprim(prim.begin(), prim.end());
+    ps.f_AB.upload(fAB.data(), fAB.size(), st);
+    ps.f_prim.upload(fprim.data(), fprim.size(), st);
+}
+
 // Shell pairs: class 0 = (s,s), 1 = (p,s), 2 = (p,p); the first shell of a
 // pair has the higher angular momentum (ties: the higher shell index).
 void Engine::setup_pairs() {
@@ -111,75 +184,9 @@ void Engine::setup_pairs() {
             if (sh[b].l > sh[a].l) std::swap(f, s);
             all[sh[f].l + sh[s].l].push_back({f, s});
         }
-    // Primitive pairs of a shell pair are stored by bound |K|/sqrt(p)
-    // descending; kpn counts those with bound * (largest bound of any pair) >=
-    // PRIM_EPS, the only ones the ERI kernels visit: a primitive quartet is
-    // at most K_ab K_cd / sqrt(p+q) times the polynomial factors of the p-type
-    // recurrences, so the dropped ones are below 1e-16 in magnitude up to
-    // those factors.  kref < 0: keep every primitive (Schwarz pass).
-    constexpr double PRIM_EPS = 1e-16;
     double bound_max = 0.0;
     auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list, double kref) {
-        ps.n = (int)list.size();
-        ps.npad = round64(std::max(ps.n, 1));
-        int kp = 1;
-        for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
-        ps.kp = kp;
-        std::vector<int> sa(ps.n), sb(ps.n), oa(ps.n), ob(ps.n), kpn(ps.npad, 0);
-        std::vector<double> AB(3 * (size_t)ps.npad, 0.0), prim((size_t)kp * PF_NFIELD * ps.npad, 0.0);
-        for (int u = 0; u < kp; ++u)  // padding entries: unit exponent, zero weight
-            for (int i = 0; i < ps.npad; ++i) {
-                prim[((size_t)u * PF_NFIELD + PF_P) * ps.npad + i] = 1.0;
-                prim[((size_t)u * PF_NFIELD + PF_O2P) * ps.npad + i] = 0.5;
-            }
-        const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
-        for (int i = 0; i < ps.n; ++i) {
-            const Shell &A = sh[list[i].first], &B = sh[list[i].second];
-            sa[i] = list[i].first; sb[i] = list[i].second; oa[i] = A.bf; ob[i] = B.bf;
-            double ab[3], r2 = 0.0;
-            for (int d = 0; d < 3; ++d) { ab[d] = A.xyz[d] - B.xyz[d]; r2 += ab[d] * ab[d]; AB[d * ps.npad + i] = ab[d]; }
-            std::vector<std::array<double, 3>> order;  // (bound, x, y)
-            for (int x = 0; x < A.np; ++x)
-                for (int y = 0; y < B.np; ++y) {
-                    const double a = A.exp[x], b = B.exp[y], p = a + b;
-                    const double K = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
-                    order.push_back({std::fabs(K) / std::sqrt(p), (double)x, (double)y});
-                }
-            std::stable_sort(order.begin(), order.end(), [](const auto &l, const auto &r) { return l[0] > r[0]; });
-            for (const auto &o : order) {
-                bound_max = std::max(bound_max, o[0]);
-                if (kref < 0.0 || o[0] * kref >= PRIM_EPS) ++kpn[i];
-            }
-            int u = 0;
-            for (const auto &o : order) {
-                const int x = (int)o[1], y = (int)o[2];
-                const double a = A.exp[x], b = B.exp[y], p = a + b;
-                double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
-                col[PF_P * ps.npad] = p;
-                for (int d = 0; d < 3; ++d) {
-                    const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
-                    col[(PF_PX + d) * ps.npad] = P;
-                    col[(PF_AX + d) * ps.npad] = P - A.xyz[d];
-                }
-                col[PF_K * ps.npad] = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
-                col[PF_O2P * ps.npad] = 0.5 / p;
-                ++u;
-            }
-            for (; u < kp; ++u) {  // padding primitives contribute zero
-                double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
-                col[PF_P * ps.npad] = 1.0;
-                col[PF_K * ps.npad] = 0.0;
-            }
-        }
-        ps.sa = sa; ps.sb = sb;
-        ps.d_sa.upload(sa.data(), ps.n, st_); ps.d_sb.upload(sb.data(), ps.n, st_);
-        ps.d_oa.upload(oa.data(), ps.n, st_); ps.d_ob.upload(ob.data(), ps.n, st_);
-        ps.d_kpn.upload(kpn.data(), kpn.size(), st_);
-        ps.d_AB.upload(AB.data(), AB.size(), st_);
-        ps.d_prim.upload(prim.data(), prim.size(), st_);
-        std::vector<float> fAB(AB.begin(), AB.end()), fprim(prim.begin(), prim.end());
-        ps.f_AB.upload(fAB.data(), fAB.size(), st_);
-        ps.f_prim.upload(fprim.data(), fprim.size(), st_);
+        fill_pairset(ps, sh, list, kref, bound_max, st_);
     };
     const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
     // pass 1: all pairs, Schwarz bounds
@@ -229,7 +236,7 @@ void Engine::setup_pairs() {
     const size_t n2 = (size_t)nbf() * nbf();
     J64_.alloc(n2); K64a_.alloc(n2); K64b_.alloc(n2); Dj64_.alloc(n2);
     Dsh64_.alloc((size_t)nshell() * nshell()); dmax64_.alloc(1);
-    count_.alloc(6);
+    count_.alloc(12);
     for (size_t q = 0; q < classes_.size(); ++q) {  // direct-mode work plans
         const int nb = std::max(pairs_[classes_[q].bra].n, 1);
         plan_nket_[q].alloc(nb); plan_off_[q].alloc(nb + 1); plan_work_[q].alloc(2);
@@ -321,7 +328,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         a.Dsh = Dsh32_.p; a.dmax = dmax32_.p;
     }
     a.J = J; a.K0 = Ka; a.K1 = uhf ? Kb : nullptr;
-    PF_HIP(hipMemsetAsync(count_.p, 0, 6 * sizeof(unsigned long long), st_));
+    PF_HIP(hipMemsetAsync(count_.p, 0, 12 * sizeof(unsigned long long), st_));
     hipEvent_t ev[7];
     if (profile_classes)
         for (auto &e : ev) PF_HIP(hipEventCreate(&e));
@@ -337,19 +344,21 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
             }
         }
         a.count = count_.p + classes_[q].slot;
+        a.pcount = count_.p + 6 + classes_[q].slot;
         a.nket = plan_nket_[q].p; a.item_off = plan_off_[q].p; a.work = plan_work_[q].p;
         if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
     if (profile_classes) PF_HIP(hipEventRecord(ev[6], st_));
     dev_assemble_g<double>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
-    unsigned long long cnt6[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long cnt6[12] = {0};
     PF_HIP(hipMemcpyAsync(cnt6, count_.p, sizeof(cnt6), hipMemcpyDeviceToHost, st_));
     PF_HIP(hipStreamSynchronize(st_));
     const int pbase = std::is_same<R, double>::value ? 0 : 6;
     unsigned long long cnt = 0;
     for (int q = 0; q < 6; ++q) {
         prof_quartets[pbase + q] += cnt6[q];
+        prof_prims[pbase + q] += cnt6[6 + q];
         prof_launches[pbase + q] += 1;
         cnt += cnt6[q];
     }

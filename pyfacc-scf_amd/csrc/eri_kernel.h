@@ -94,13 +94,25 @@ __device__ __forceinline__ void eri_block(const PairView<R> &B, const R *__restr
     E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, eri);
 }
 
+// Densities a digest reads: Dj for J (the total density), Dk0/Dk1 for K.
+template <typename R>
+struct DensView {
+    const R *Dj, *Dk0, *Dk1;
+    int nbf;
+};
+template <typename R>
+__device__ __forceinline__ DensView<R> dens_view(const JKArgs<R> &g) {
+    return DensView<R>{g.Dj, g.Dk0, g.Dk1, g.nbf};
+}
+
 // J/K digest of one quartet (bra functions oa.., ob.., ket functions oc..,
-// od..).  jab[NA*NB] accumulates J'_ab (reduced by the caller); J'_cd goes
-// to device memory; kadd(kk, r, col, v) adds v to K'(row r, col) of density
-// kk, bra row r < NA being function oa + r and r >= NA function ob + r - NA.
-template <int LA, int LB, int LC, int LD, typename R, int NK, typename KAdd>
-__device__ __forceinline__ void digest(const JKArgs<R> &g, const R *eri, int oa, int ob, int oc, int od, R deg,
-                                       const R *dab, R *jab, KAdd &&kadd) {
+// od..).  jab[NA*NB] accumulates J'_ab (reduced by the caller);
+// jadd(row, col, v) adds v to J'(row, col) for the ket block;
+// kadd(kk, r, col, v) adds v to K'(row r, col) of density kk, bra row r < NA
+// being function oa + r and r >= NA function ob + r - NA.
+template <int LA, int LB, int LC, int LD, typename R, int NK, typename JAdd, typename KAdd>
+__device__ __forceinline__ void digest(const DensView<R> &g, const R *eri, int oa, int ob, int oc, int od, R deg,
+                                       const R *dab, R *jab, JAdd &&jadd, KAdd &&kadd) {
     using E = Eri<LA, LB, LC, LD, R>;
     constexpr int NA = E::NA, NB = E::NB, NC = E::NC, ND = E::ND;
     const int nbf = g.nbf;
@@ -125,7 +137,7 @@ __device__ __forceinline__ void digest(const JKArgs<R> &g, const R *eri, int oa,
             R s = R(0);
 #pragma unroll
             for (int ab = 0; ab < NA * NB; ++ab) s += eri[ab * NC * ND + ic * ND + id] * dab[ab];
-            atomicAdd(&g.J[(size_t)(oc + ic) * nbf + od + id], double(deg2 * s));
+            jadd(oc + ic, od + id, deg2 * s);
         }
     // ---- exchange, one pass per density
 #pragma unroll
@@ -220,6 +232,8 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
     const int lane = threadIdx.x & 63;
     const int nbf = g.nbf;
     const int nket = MODE == MODE_PACKED ? (g.same ? i + 1 : K.n) : g.store_nket[i];
+    // batched stores: bra i's kets start at store_kbase[i] of the ket list
+    const int kbase = (MODE == MODE_STORE && g.store_kbase) ? g.store_kbase[i] : 0;
     const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
     const R ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
     const R deg_ab = (a == b) ? R(0.5) : R(1);
@@ -233,11 +247,12 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
                 jab[ia * NB + ib] = R(0);
             }
     }
-    unsigned long long counted = 0;
+    unsigned long long counted = 0, pcounted = 0;
     for (int j0 = 0; j0 < nket; j0 += 64) {
-        const int j = j0 + lane;
-        if (j >= nket) break;
+        if (j0 + lane >= nket) break;
+        const int j = kbase + j0 + lane;
         ++counted;
+        if constexpr (COMPUTE) pcounted += (unsigned long long)K.kpn[j] * B.kpn[i];
         R eri[NINT];
         if constexpr (MODE == MODE_LOAD) {
             const R *src = g.store + g.store_off[i] + (size_t)j0 * NINT + lane;
@@ -250,6 +265,11 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
             R *dst = g.store + g.store_off[i] + (size_t)j0 * NINT + lane;
 #pragma unroll
             for (int k = 0; k < NINT; ++k) dst[k * 64] = eri[k];
+            if (g.store32) {
+                float *d32 = g.store32 + g.store_off[i] + (size_t)j0 * NINT + lane;
+#pragma unroll
+                for (int k = 0; k < NINT; ++k) d32[k * 64] = float(eri[k]);
+            }
         } else if constexpr (MODE == MODE_PACKED) {
             const int oc = K.oa[j], od = K.ob[j];
 #pragma unroll
@@ -265,11 +285,14 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
         } else {
             const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
-            digest<LA, LB, LC, LD, R, NK>(g, eri, oa, ob, oc, od, deg, dab, jab, [&](int kk, int r, int col, R v) {
-                double *Ko = kk == 0 ? g.K0 : g.K1;
-                const int row = r < NA ? oa + r : ob + r - NA;
-                atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
-            });
+            digest<LA, LB, LC, LD, R, NK>(
+                dens_view(g), eri, oa, ob, oc, od, deg, dab, jab,
+                [&](int row, int col, R v) { atomicAdd(&g.J[(size_t)row * nbf + col], double(v)); },
+                [&](int kk, int r, int col, R v) {
+                    double *Ko = kk == 0 ? g.K0 : g.K1;
+                    const int row = r < NA ? oa + r : ob + r - NA;
+                    atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
+                });
         }
     }
     if constexpr (MODE == MODE_LOAD) {
@@ -280,10 +303,16 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
         }
     }
     if (g.count) {
-        unsigned long long tot = counted;
+        unsigned long long tot = counted, ptot = pcounted;
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
-        if (lane == 0) atomicAdd(g.count, tot);
+        for (int o = 32; o >= 1; o >>= 1) {
+            tot += __shfl_xor(tot, o, 64);
+            ptot += __shfl_xor(ptot, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(g.count, tot);
+            if (g.pcount) atomicAdd(g.pcount, ptot);
+        }
     }
 }
 
@@ -318,7 +347,7 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
     int *list = lds_list[wave];
     const bool compact = g.Dsh != nullptr;
     const int total = g.work[1];
-    unsigned long long counted = 0;
+    unsigned long long counted = 0, pcounted = 0;
     __syncthreads();
     while (true) {
         if (threadIdx.x == 0) s_item = atomicAdd(&g.work[0], 1);
@@ -337,7 +366,7 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
         const int wb = min(jend, jbeg + wave * per), we = min(jend, wb + per);
 
         const R Qi = B.Q[i];
-        const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
+        const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i], kpn_i = B.kpn[i];
         const R ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
         const R deg_ab = (a == b) ? R(0.5) : R(1);
         R dab[NA * NB], jab[NA * NB];
@@ -390,19 +419,23 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
             if (j < 0) continue;
             const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
             ++counted;
+            pcounted += (unsigned long long)K.kpn[j] * kpn_i;
             R eri[NINT];
             eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
+            const auto jadd = [&](int row, int col, R v) { atomicAdd(&g.J[(size_t)row * nbf + col], double(v)); };
             if (g.krows) {
-                digest<LA, LB, LC, LD, R, NK>(g, eri, oa, ob, oc, od, deg, dab, jab, [&](int kk, int r, int col, R v) {
-                    atomicAdd(&lds_acc[(kk * NAB + r) * nbf + col], double(v));
-                });
+                digest<LA, LB, LC, LD, R, NK>(dens_view(g), eri, oa, ob, oc, od, deg, dab, jab, jadd,
+                                              [&](int kk, int r, int col, R v) {
+                                                  atomicAdd(&lds_acc[(kk * NAB + r) * nbf + col], double(v));
+                                              });
             } else {
-                digest<LA, LB, LC, LD, R, NK>(g, eri, oa, ob, oc, od, deg, dab, jab, [&](int kk, int r, int col, R v) {
-                    double *Ko = kk == 0 ? g.K0 : g.K1;
-                    const int row = r < NA ? oa + r : ob + r - NA;
-                    atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
-                });
+                digest<LA, LB, LC, LD, R, NK>(dens_view(g), eri, oa, ob, oc, od, deg, dab, jab, jadd,
+                                              [&](int kk, int r, int col, R v) {
+                                                  double *Ko = kk == 0 ? g.K0 : g.K1;
+                                                  const int row = r < NA ? oa + r : ob + r - NA;
+                                                  atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
+                                              });
             }
         }
 #pragma unroll
@@ -428,10 +461,16 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
         __syncthreads();
     }
     if (g.count) {
-        unsigned long long tot = counted;
+        unsigned long long tot = counted, ptot = pcounted;
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
-        if (lane == 0) atomicAdd(g.count, tot);
+        for (int o = 32; o >= 1; o >>= 1) {
+            tot += __shfl_xor(tot, o, 64);
+            ptot += __shfl_xor(ptot, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(g.count, tot);
+            if (g.pcount) atomicAdd(g.pcount, ptot);
+        }
     }
 }
 

@@ -50,10 +50,13 @@ struct JKArgs {
     R *store = nullptr;
     const long long *store_off = nullptr;  // per bra: first element in store
     const int *store_nket = nullptr;       // per bra: number of stored kets
+    const int *store_kbase = nullptr;      // per bra: first ket (batched stores), nullptr = 0
+    float *store32 = nullptr;              // optional FP32 copy of the store (same layout)
     // packed canonical export
     double *packed = nullptr;
     // statistics (quartets that passed screening), may be nullptr
     unsigned long long *count = nullptr;
+    unsigned long long *pcount = nullptr;  // primitive quartets evaluated, may be nullptr
     // direct-mode work plan (device): per bra ket-prefix length [bra.n],
     // first work item [bra.n + 1], queue head and item total [2]; kets per item
     int *nket = nullptr, *item_off = nullptr, *work = nullptr;

@@ -58,6 +58,10 @@ struct Basis {
 // Basis construction (basis.cpp).
 Basis build_basis(const std::string &path, const std::vector<std::string> &species, const Molecule &mol,
                   bool renormalise);
+// The same from the file's lines, read once by read_basis_file (batched use).
+std::vector<std::string> read_basis_file(const std::string &path);
+Basis build_basis_lines(const std::vector<std::string> &lines, const std::string &path,
+                        const std::vector<std::string> &species, const Molecule &mol, bool renormalise);
 std::string resolve_basis_path(const std::string &name);
 double nuclear_repulsion(const Molecule &mol);
 

@@ -14,6 +14,7 @@ from ._native import (  # noqa: F401
     Result,
     PyfaccError,
     basis_info,
+    batch_scf,
     boys,
     default_options,
     device_count,

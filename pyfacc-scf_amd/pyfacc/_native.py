@@ -52,7 +52,7 @@ _SIGS = {
     "pf_eri_packed": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd]),
     "pf_jk": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pd, _pd, _pd, _i, _i, _d]),
     "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
-    "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i)]),
+    "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i), _P(ctypes.c_ulonglong)]),
     "pf_scf_gradient": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd,
                              _pd, _pd]),
     "pf_boys": (_i, [_i, _i, _pd, _pd]),
@@ -60,6 +60,8 @@ _SIGS = {
     "pf_projector": (_i, [_i, _pd, _d, _pd]),
     "pf_gemm": (_i, [_i, _i, _i, _i, _i, _d, _pd, _pd, _d, _pd]),
     "pf_device_copy": (_i, [ctypes.c_ulonglong, _pd]),
+    "pf_batch_scf": (_i, [_i, _pi32, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _pi32, _pi32,
+                          ctypes.c_void_p, _i, _pd, _pi32, _pi32, ctypes.c_void_p]),
 }
 
 
@@ -192,16 +194,57 @@ def scf(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, want_
     return out
 
 
-def scf_profile():
+class BatchStats(ctypes.Structure):
+    _fields_ = [("t_total", ctypes.c_double), ("t_setup", ctypes.c_double), ("t_eri", ctypes.c_double),
+                ("t_scf", ctypes.c_double), ("eri_values", ctypes.c_double), ("store_bytes", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("graph", ctypes.c_int), ("digest_waves", ctypes.c_int),
+                ("nproblems", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def batch_scf(molecules, basis="sto-3g", graph=True, options=None, **kw):
+    """Batched SCF of many molecules (``pf_batch_scf``).
+
+    molecules: sequence of (species, coords_bohr (natoms, 3), nalpha, nbeta).
+    Returns a dict with per-molecule ``energy``, ``iterations``, ``converged``
+    arrays and the call's statistics under ``stats``."""
+    opt = options if options is not None else default_options(**kw)
+    mols = [_Mol(sp, xyz) for sp, xyz, _, _ in molecules]
+    nmol = len(mols)
+    natoms = np.array([m.n for m in mols], dtype=np.int32)
+    species = b"".join(m.sp for m in mols)
+    z = np.ascontiguousarray(np.concatenate([m.z for m in mols]).astype(np.int32))
+    xyz = np.ascontiguousarray(np.concatenate([m.xyz for m in mols]).astype(np.float64))
+    na = np.array([int(t[2]) for t in molecules], dtype=np.int32)
+    nb = np.array([int(t[3]) for t in molecules], dtype=np.int32)
+    e = np.zeros(nmol)
+    it = np.zeros(nmol, dtype=np.int32)
+    cv = np.zeros(nmol, dtype=np.int32)
+    st = BatchStats()
+    _check(load_library().pf_batch_scf(nmol, natoms.ctypes.data_as(_pi32), species, z.ctypes.data_as(_pi32),
+                                       xyz.ctypes.data_as(_pd), basis.encode(), na.ctypes.data_as(_pi32),
+                                       nb.ctypes.data_as(_pi32), ctypes.cast(ctypes.byref(opt), ctypes.c_void_p),
+                                       int(bool(graph)), e.ctypes.data_as(_pd), it.ctypes.data_as(_pi32),
+                                       cv.ctypes.data_as(_pi32), ctypes.cast(ctypes.byref(st), ctypes.c_void_p)))
+    return {"energy": e, "iterations": it, "converged": cv.astype(bool), "stats": st.as_dict()}
+
+
+def scf_profile(with_prims=False):
     """(quartets, ms, launches) each shaped (2, 6) = [FP64, FP32] x class, and
-    the (6, 4) angular momenta (la lb lc ld) of each class as launched."""
+    the (6, 4) angular momenta (la lb lc ld) of each class as launched; with
+    with_prims also the (2, 6) primitive quartets evaluated."""
     q = (ctypes.c_ulonglong * 12)()
+    pq = (ctypes.c_ulonglong * 12)()
     ms = np.zeros(12)
     n = np.zeros(12, dtype=np.int32)
     cl = np.zeros(24, dtype=np.int32)
-    load_library().pf_scf_profile(q, _ptr(ms), n.ctypes.data_as(_P(_i)), cl.ctypes.data_as(_P(_i)))
-    return (np.array(list(q), dtype=np.int64).reshape(2, 6), ms.reshape(2, 6), n.reshape(2, 6),
-            cl.reshape(6, 4))
+    load_library().pf_scf_profile(q, _ptr(ms), n.ctypes.data_as(_P(_i)), cl.ctypes.data_as(_P(_i)), pq)
+    out = (np.array(list(q), dtype=np.int64).reshape(2, 6), ms.reshape(2, 6), n.reshape(2, 6), cl.reshape(6, 4))
+    if with_prims:
+        out = out + (np.array(list(pq), dtype=np.int64).reshape(2, 6),)
+    return out
 
 
 def device_copy(nbytes):

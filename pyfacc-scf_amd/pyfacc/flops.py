@@ -117,15 +117,18 @@ CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0),
 CLASS_NAMES = ["(ss|ss)", "(ps|ss)", "(ps|ps)", "(pp|ss)", "(pp|ps)", "(pp|pp)"]
 
 
-def class_flops(cls_index, quartets, kprim_pair=9, ndens=1, taylor_terms=7, orient=None):
+def class_flops(cls_index, quartets, kprim_pair=9, ndens=1, taylor_terms=7, orient=None, prim_quartets=None):
     """FLOPs of ``quartets`` quartets of class CLASSES[cls_index] with
-    ``kprim_pair`` primitive pairs per shell pair (9 for STO-3G); ``orient``
-    gives (la, lb, lc, ld) as launched when bra and ket were swapped."""
+    ``kprim_pair`` primitive pairs per shell pair (9 for STO-3G), or with
+    ``prim_quartets`` primitive quartets in total when given (the kernels skip
+    primitive pairs whose bound is negligible); ``orient`` gives
+    (la, lb, lc, ld) as launched when bra and ket were swapped."""
     c = tuple(int(x) for x in orient) if orient is not None else CLASSES[cls_index]
-    return quartets * (kprim_pair * kprim_pair * per_primitive(*c, taylor_terms=taylor_terms)
-                       + per_contracted(*c, ndens=ndens))
+    nprim = quartets * kprim_pair * kprim_pair if prim_quartets is None else prim_quartets
+    return nprim * per_primitive(*c, taylor_terms=taylor_terms) + quartets * per_contracted(*c, ndens=ndens)
 
 
-def fock_flops(class_quartets, kprim_pair=9, ndens=1, taylor_terms=7, orients=None):
-    return sum(class_flops(i, q, kprim_pair, ndens, taylor_terms, None if orients is None else orients[i])
+def fock_flops(class_quartets, kprim_pair=9, ndens=1, taylor_terms=7, orients=None, class_prims=None):
+    return sum(class_flops(i, q, kprim_pair, ndens, taylor_terms, None if orients is None else orients[i],
+                           None if class_prims is None else class_prims[i])
                for i, q in enumerate(class_quartets))

@@ -1,0 +1,96 @@
+"""Batched SCF (pf_batch_scf) over seeded water clusters of 1-12 molecules,
+some with an OH radical (UHF doublet).
+
+Every molecule's batched energy is checked against the single-molecule GPU
+engine (pf_scf, direct J/K) and, for a few, against the CPU oracle; the
+hipGraph replay against eager launches (several replays with host
+synchronisation between them, each compared); the mixed form against FP64 at
+the 1e-6 Eh bar.
+"""
+import numpy as np
+import pytest
+
+from conftest import ANG2BOHR
+
+pytestmark = pytest.mark.gpu
+
+# instances: k = 1, 2, 3, 5, 6, 11 waters and OH-substituted ones (k = 3, 4, 7)
+IDS = [3, 1, 4, 2, 0, 11, 59, 35, 12]
+OPTS = dict(eps=1e-9, max_iter=100, diis=8, schwarz=1e-11)
+
+
+def _instances(ids):
+    from pyfacc import clusters
+    out = []
+    for i in ids:
+        sp, xyz, k, oh = clusters.batched_instance(i)
+        ne = sum({"H": 1, "O": 8}[s] for s in sp)
+        na, nb = (ne + 1) // 2, ne // 2
+        out.append((sp, np.asarray(xyz) * ANG2BOHR, na, nb))
+    return out
+
+
+@pytest.fixture(scope="module")
+def batch64(gpu_lib):
+    import pyfacc
+    mols = _instances(IDS)
+    return mols, pyfacc.batch_scf(mols, graph=True, **OPTS)
+
+
+def test_batch_matches_single_molecule_engine(batch64):
+    import pyfacc
+    mols, r = batch64
+    assert r["converged"].all()
+    for (sp, xyz, na, nb), e in zip(mols, r["energy"]):
+        ref = pyfacc.scf(sp, xyz, na, nb, **OPTS)
+        assert ref["converged"]
+        assert abs(e - ref["energy"]) < 1e-8, (len(sp), na, nb)
+
+
+def test_batch_matches_oracle(batch64):
+    import oracle
+    mols, r = batch64
+    for idx in (0, 2, 6):  # one water, three waters, three waters + OH (UHF)
+        sp, xyz, na, nb = mols[idx]
+        ref = oracle.Molecule(sp, xyz).scf(na, nb, eps=1e-10, max_iter=150, diis=8)
+        assert ref["converged"]
+        assert abs(r["energy"][idx] - ref["energy"]) < 1e-8
+
+
+def test_batch_graph_replay_matches_eager(gpu_lib, batch64):
+    import pyfacc
+    mols, r = batch64
+    for _ in range(2):
+        e = pyfacc.batch_scf(mols, graph=False, **OPTS)
+        assert e["converged"].all()
+        assert np.abs(e["energy"] - r["energy"]).max() < 1e-10
+        # LDS atomics sum in no fixed order: the stopping test may trip one step apart
+        assert np.abs(e["iterations"] - r["iterations"]).max() <= 2
+    g = pyfacc.batch_scf(mols, graph=True, **OPTS)
+    assert g["stats"]["graph"] == 1
+    assert np.abs(g["energy"] - r["energy"]).max() < 1e-10
+
+
+def test_batch_mixed_within_bar(gpu_lib, batch64):
+    import pyfacc
+    mols, r = batch64
+    m = pyfacc.batch_scf(mols, graph=True, precision=1, **OPTS)
+    assert m["converged"].all()
+    assert np.abs(m["energy"] - r["energy"]).max() < 1e-6
+
+
+def test_batch_plain_roothaan_and_errors(gpu_lib):
+    import pyfacc
+    mols = _instances([3, 7, 9])  # three single waters
+    r = pyfacc.batch_scf(mols, eps=1e-10, max_iter=100, diis=0, schwarz=1e-12)
+    assert r["converged"].all()
+    for (sp, xyz, na, nb), e, it in zip(mols, r["energy"], r["iterations"]):
+        ref = pyfacc.scf(sp, xyz, na, nb, eps=1e-10, max_iter=100, diis=0, schwarz=1e-12)
+        assert abs(e - ref["energy"]) < 1e-9
+        assert abs(int(it) - ref["iterations"]) <= 1
+    # an unconverged batch reports it; an impossible electron count raises
+    r = pyfacc.batch_scf(mols, eps=1e-12, max_iter=2, diis=0)
+    assert not r["converged"].any() and (r["iterations"] == 2).all()
+    sp, xyz, na, nb = mols[0]
+    with pytest.raises(pyfacc.PyfaccError):
+        pyfacc.batch_scf([(sp, xyz, 40, 40)])
