@@ -105,56 +105,75 @@ Basis build_basis(const std::string &path, const std::vector<std::string> &speci
     return build_basis_lines(read_basis_file(path), path, species, mol, renormalise);
 }
 
-Basis build_basis_lines(const std::vector<std::string> &lines, const std::string &path,
-                        const std::vector<std::string> &species, const Molecule &mol, bool renormalise) {
-    Basis basis;
+BasisBlocks parse_basis_blocks(const std::vector<std::string> &lines, const std::string &path) {
+    BasisBlocks blocks;
+    blocks.path = path;
     size_t i = 0;
     while (i < lines.size()) {
         std::string line = lines[i++];
         if (trim(line).empty() || line[0] == '!') continue;
         auto tok = split(line);
         if (tok.empty() || tok[0] == "****") continue;
-        const std::string label = tok[0];
-        std::vector<int> atoms;
-        for (int a = 0; a < (int)species.size(); ++a)
-            if (species[a] == label) atoms.push_back(a);
-        // walk the block to its terminator either way
+        BasisBlocks::Block blk;
+        blk.label = tok[0];
+        // walk the block to its terminator
         while (i < lines.size()) {
             std::string sl = lines[i++];
             if (sl.find("****") != std::string::npos) break;
             auto st = split(sl);
             if (st.size() < 2) continue;
-            const std::string kind = st[0];
-            int np = std::atoi(st[1].c_str());
-            if (np <= 0 || np > KMAX) throw Error("basis file: bad primitive count on line '" + sl + "'");
-            std::vector<std::vector<double>> rows;
-            for (int k = 0; k < np; ++k) {
-                if (i >= lines.size()) throw Error("basis file: truncated shell for " + label);
+            BasisBlocks::ShellLine sh;
+            sh.kind = st[0];
+            sh.np = std::atoi(st[1].c_str());
+            if (sh.np <= 0 || sh.np > KMAX) throw Error("basis file: bad primitive count on line '" + sl + "'");
+            for (int k = 0; k < sh.np; ++k) {
+                if (i >= lines.size()) throw Error("basis file: truncated shell for " + blk.label);
                 std::vector<double> r;
                 for (auto &t : split(lines[i++])) r.push_back(parse_real(t));
-                rows.push_back(r);
+                sh.rows.push_back(r);
             }
-            if (atoms.empty()) continue;
+            blk.shells.push_back(sh);
+        }
+        blocks.blocks.push_back(blk);
+    }
+    return blocks;
+}
+
+Basis build_basis_lines(const std::vector<std::string> &lines, const std::string &path,
+                        const std::vector<std::string> &species, const Molecule &mol, bool renormalise) {
+    return build_basis_blocks(parse_basis_blocks(lines, path), species, mol, renormalise);
+}
+
+Basis build_basis_blocks(const BasisBlocks &blocks, const std::vector<std::string> &species, const Molecule &mol,
+                         bool renormalise) {
+    Basis basis;
+    for (const auto &blk : blocks.blocks) {
+        std::vector<int> atoms;
+        for (int a = 0; a < (int)species.size(); ++a)
+            if (species[a] == blk.label) atoms.push_back(a);
+        if (atoms.empty()) continue;
+        for (const auto &sh : blk.shells) {
             auto add = [&](int l, int col) {
                 for (int a : atoms) {
                     Shell s;
                     s.l = l;
-                    s.np = np;
+                    s.np = sh.np;
                     s.atom = a;
                     for (int d = 0; d < 3; ++d) s.xyz[d] = mol.xyz[3 * a + d];
-                    for (int k = 0; k < np; ++k) {
-                        if ((int)rows[k].size() <= col) throw Error("basis file: missing coefficient for " + label);
-                        s.exp[k] = rows[k][0];
-                        s.coef[k] = rows[k][col];
+                    for (int k = 0; k < sh.np; ++k) {
+                        if ((int)sh.rows[k].size() <= col)
+                            throw Error("basis file: missing coefficient for " + blk.label);
+                        s.exp[k] = sh.rows[k][0];
+                        s.coef[k] = sh.rows[k][col];
                     }
                     basis.shells.push_back(s);
                 }
             };
-            if (kind == "S") add(0, 1);
-            else if (kind == "P") add(1, 1);
-            else if (kind == "SP") { add(0, 1); add(1, 2); }
-            else if (!atoms.empty())
-                throw Error("basis file: shell type '" + kind + "' of " + label +
+            if (sh.kind == "S") add(0, 1);
+            else if (sh.kind == "P") add(1, 1);
+            else if (sh.kind == "SP") { add(0, 1); add(1, 2); }
+            else
+                throw Error("basis file: shell type '" + sh.kind + "' of " + blk.label +
                             " is not supported (s and p shells only)");
         }
     }
@@ -162,7 +181,7 @@ Basis build_basis_lines(const std::vector<std::string> &lines, const std::string
     std::vector<int> seen(species.size(), 0);
     for (auto &s : basis.shells) seen[s.atom] = 1;
     for (size_t a = 0; a < species.size(); ++a)
-        if (!seen[a]) throw Error("basis file " + path + " has no functions for element '" + species[a] + "'");
+        if (!seen[a]) throw Error("basis file " + blocks.path + " has no functions for element '" + species[a] + "'");
 
     int bf = 0;
     for (auto &s : basis.shells) {
@@ -184,8 +203,6 @@ Basis build_basis_lines(const std::vector<std::string> &lines, const std::string
     return basis;
 }
 
-// E_nuc = sum_{i<j} Z_i Z_j / R_ij, pairs closer than 1e-10 bohr skipped
-// (molecule.f90:35-49).
 double nuclear_repulsion(const Molecule &mol) {
     double e = 0.0;
     for (int i = 0; i < mol.natoms; ++i)

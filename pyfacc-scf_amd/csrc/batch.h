@@ -6,7 +6,8 @@
 // "problems": one per RHF molecule, two per UHF one):
 //   digest (one workgroup per molecule, J/K in LDS) -> F = H + G,
 //   DIIS (error FDS - SDF, B matrix, per-molecule solve, extrapolation),
-//   F' = X^T F X, SP2 purification, D = occ X P X^T, energy and convergence.
+//   F' = X^T F X, SP2 purification (one workgroup per problem, steps until
+//   its own convergence), D = occ X P X^T, energy and convergence.
 // Converged molecules are frozen.  The sequence contains no host decision, so
 // it is captured once into a hipGraph and replayed every iteration; the host
 // reads one counter (molecules still running) per iteration.
@@ -39,15 +40,23 @@ struct BatchClass {
     int same = 0;
 };
 
+// A digest work item: molecule m's bra pairs from (class q0, pair i0) to
+// (class q1, pair i1) exclusive; split = the molecule has several items.
+struct BatchItem {
+    int m, q0, i0, q1, i1, split;
+};
+
 // Per-molecule and per-problem state shared by the batched kernels.
 struct BatchDev {
     int M = 0, P = 0, nmax = 0, ndiis = 0;
     const int *mn = nullptr, *muhf = nullptr, *mprob = nullptr;  // nbf, UHF flag, first problem
     const long long *moff = nullptr;                              // molecule matrix offsets
+    const long long *gacc_off = nullptr;                          // split molecules' J/K sums
     const int *pn = nullptr, *pmol = nullptr;
     const double *poccf = nullptr, *pnocc = nullptr;              // occupation factor, occupied count
     const long long *poff = nullptr, *pmoff = nullptr;            // problem / its molecule's matrix offsets
-    const long long *pdiis = nullptr, *psp2 = nullptr;            // DIIS ring and SP2 state offsets
+    const long long *pdiis = nullptr;                             // DIIS ring offsets
+    const long long *psp2 = nullptr;  // Gershgorin scratch offsets (orthogonaliser's molecule view)
     int *mactive = nullptr;   // 1 while the molecule iterates
     int *mprec = nullptr;     // mixed form: 0 = FP32 integrals, 1 = FP64
     int *mhead = nullptr, *mnstore = nullptr, *muse = nullptr;  // DIIS ring head / size / extrapolate flag
@@ -84,20 +93,22 @@ class BatchEngine {
     bool renorm_;
     double schwarz_;
     double t_eri_ = 0.0;
+    bool verbose_ = false;
     int M_ = 0, P_ = 0, nmax_ = 0, ndiis_ = 8;
     std::vector<int> mn_, muhf_, mprob_, pn_, pmol_;
     std::vector<double> enuc_;
     std::vector<long long> moff_, poff_;
     long long mtot_ = 0, ptot_ = 0;
     std::vector<Shell> shells_;
+    DevBuf<Shell> d_shells_;
     // device
     DevBuf<double> tab_;
     DevBuf<int> d_mn_, d_muhf_, d_mprob_, d_pn_, d_pmol_, d_mactive_, d_mprec_, d_mhead_, d_mnstore_, d_muse_;
     DevBuf<int> d_count_;
     DevBuf<double> d_poccf_, d_pnocc_, d_enuc_, d_E_, d_Eold_, d_iters_, d_conv_, d_coef_, d_B_;
-    DevBuf<long long> d_moff_, d_poff_, d_pmoff_, d_pdiis_, d_psp2_;
+    DevBuf<long long> d_moff_, d_poff_, d_pmoff_, d_pdiis_;
     DevBuf<double> S_, H_, X_, Dtot_;                    // molecule pools
-    DevBuf<double> D_, Dold_, F_, Fx_, Fp_, W1_, W2_, Pr_, sp2_;  // problem pools
+    DevBuf<double> D_, Dold_, F_, Fx_, Fp_, W1_, W2_, Pr_;  // problem pools
     DevBuf<double> dF_, dE_;                             // DIIS rings
     std::array<PairSet, 3> pairs_;
     std::array<std::vector<int>, 3> pbeg_;  // per class: molecule m's pairs pbeg[m] .. pbeg[m+1]-1
@@ -107,6 +118,11 @@ class BatchEngine {
     std::array<DevBuf<long long>, 6> soff_;
     std::array<DevBuf<int>, 6> snket_, skbase_, sbr_;
     std::array<BatchClass, 6> cls_;
+    DevBuf<BatchItem> items_;
+    DevBuf<int> d_msplit_;
+    DevBuf<long long> d_gacc_off_;
+    DevBuf<double> gacc_;
+    int nitems_ = 0;
     BatchDev dev_;
     double eri_values_ = 0;
     int tuned_waves_ = 0;

@@ -27,8 +27,9 @@ namespace pf {
 
 size_t batch_digest_lds(int nmax);
 void batch_digest_prepare(int nmax);
-void launch_batch_digest(const BatchDev &b, const std::array<BatchClass, 6> &cls, const double *D,
-                         const double *Dtot, const double *H, double *F, int waves, hipStream_t st);
+void launch_batch_digest(const BatchDev &b, const std::array<BatchClass, 6> &cls, const BatchItem *items,
+                         int nitems, const int *msplit, const double *D, const double *Dtot, const double *H,
+                         double *F, double *gacc, int waves, hipStream_t st);
 
 namespace {
 
@@ -38,7 +39,7 @@ double now() {
 __host__ __device__ inline int ntile(int n) { return ((n + 31) / 32) * ((n + 31) / 32); }
 long long round64(long long n) { return (n + 63) / 64 * 64; }
 size_t sp2_size(int n) { return 8 + 6 * (size_t)ntile(n) + 2 * (size_t)n; }
-constexpr int NSP2 = 64;  // purification steps per density (even: the result ends in the first buffer)
+constexpr int SP2_MAX_STEPS = 100;  // purification steps per density at most (converged ones stop early)
 
 // C_p = alpha a_p op(A_p) op(B_p) + beta Cin_p for each problem p (a_p = ascale[p], or 1).
 struct BGemm {
@@ -82,13 +83,6 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BatchDev b, BGemm g) {
 // dst_p = src_p (src offsets per problem: os)
 __global__ void __launch_bounds__(256) bcopy_kernel(BatchDev b, const double *src, const long long *os, double *dst) {
     PROBLEM_LOOP(b) dst[b.poff[p] + e] = src[os[p] + e];
-}
-// dst_p = (A_p + A_p^T) / 2
-__global__ void __launch_bounds__(256) bsym_kernel(BatchDev b, const double *A, double *dst) {
-    PROBLEM_LOOP(b) {
-        const int r = e / n, c = e - r * n;
-        dst[b.poff[p] + e] = 0.5 * (A[b.poff[p] + e] + A[b.poff[p] + (size_t)c * n + r]);
-    }
 }
 // DIIS insert at the molecule's ring head: E = W - W^T (W = F D S), and F
 __global__ void __launch_bounds__(256) bdiis_insert_kernel(BatchDev b, const double *W, const double *F, double *dE,
@@ -183,7 +177,97 @@ __global__ void __launch_bounds__(256) bdtot_kernel(BatchDev b, const double *D,
     }
 }
 
-// ---- SP2 per problem (state layout as sp2_density in linalg.hip)
+// ---- SP2 per problem: one workgroup runs the whole purification of its
+// problem (Gershgorin bounds, X0 = (hi I - F')/(hi - lo), then X <- X^2 or
+// 2X - X^2 chosen from tr X and tr X^2 = sum X_ij^2 until the idempotency
+// error |tr X - tr X^2| is below tol or stops shrinking below 1e-8), as
+// sp2_density does for one matrix with one launch per step.  The iterates
+// ping-pong between Xa and Xb; the result, symmetrised, goes to P.
+__device__ __forceinline__ double block_min256(double v, double *sh) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double t = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+    __syncthreads();
+    return t;
+}
+__global__ void __launch_bounds__(256) bsp2_kernel(BatchDev b, const double *Fp, double *Xa, double *Xb, double *Pout,
+                                                    int max_steps) {
+    __shared__ double red[4][16][64];
+    __shared__ double sh[4];
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]]) return;
+    const size_t o = b.poff[p];
+    const double *F = Fp + o;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // Gershgorin discs, one wavefront per row
+    double lo = 1e300, hi = -1e300;
+    for (int i = w; i < n; i += 4) {
+        double r = 0.0;
+        for (int j = lane; j < n; j += 64)
+            if (j != i) r += fabs(F[(size_t)i * n + j]);
+        for (int s = 32; s >= 1; s >>= 1) r += __shfl_xor(r, s, 64);
+        const double d = F[(size_t)i * n + i];
+        lo = fmin(lo, d - r);
+        hi = fmax(hi, d + r);
+    }
+    lo = block_min256(lo, sh);
+    hi = -block_min256(-hi, sh);
+    const double sc = 1.0 / (hi - lo);
+    double *cur = Xa + o, *nxt = Xb + o;
+    double t1 = 0.0, t2 = 0.0;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int r = e / n, c = e - r * n;
+        const double v = ((r == c ? hi : 0.0) - F[e]) * sc;
+        cur[e] = v;
+        t2 += v * v;
+        if (r == c) t1 += v;
+    }
+    t1 = block_sum256(t1, sh);
+    t2 = block_sum256(t2, sh);
+    const double nocc = b.pnocc[p], tol = 1e-13 * fmax(1.0, nocc);
+    const int nt = (n + 31) / 32;
+    double prev = 1e300;
+    for (int k = 0; k < max_steps; ++k) {
+        const double err = fabs(t1 - t2);
+        if (k > 2 && (err < tol || (err < 1e-8 && err >= prev))) break;
+        prev = err;
+        const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
+        double u1 = 0.0, u2 = 0.0;
+        for (int tile = 0; tile < nt * nt; ++tile) {
+            const int r0 = (tile / nt) * 32, c0 = (tile % nt) * 32;
+            mfma_tile(n, n, n, 0, 0, cur, n, cur, n, r0, c0, red);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
+                int gr, gc;
+                tile_pos(idx, r0, c0, gr, gc);
+                if (gr < n && gc < n) {
+                    const size_t e = (size_t)gr * n + gc;
+                    const double x2 = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+                    const double v = square ? x2 : 2.0 * cur[e] - x2;
+                    nxt[e] = v;
+                    u2 += v * v;
+                    if (gr == gc) u1 += v;
+                }
+            }
+            __syncthreads();  // red is rewritten by the next tile
+        }
+        t1 = block_sum256(u1, sh);  // also orders this step's writes before the next step's reads
+        t2 = block_sum256(u2, sh);
+        double *t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    double *P = Pout + o;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int r = e / n, c = e - r * n;
+        P[e] = 0.5 * (cur[e] + cur[(size_t)c * n + r]);
+    }
+}
+
+// Gershgorin discs of problem matrices (the orthogonaliser's molecule view):
+// rows and bounds into the per-problem state at psp2.
 __global__ void __launch_bounds__(256) bgersh_rows_kernel(BatchDev b, const double *A, double *st) {
     const int p = blockIdx.x, n = b.pn[p];
     if (!b.mactive[b.pmol[p]]) return;
@@ -216,91 +300,6 @@ __global__ void __launch_bounds__(256) bgersh_bounds_kernel(BatchDev b, double *
         for (int w = 0; w < 4; ++w) { lo = fmin(lo, lo_s[w]); hi = fmax(hi, hi_s[w]); }
         s[0] = lo;
         s[1] = hi;
-    }
-}
-__global__ void __launch_bounds__(256) bsp2_init_kernel(BatchDev b, const double *Fp, double *st, double *X) {
-    __shared__ double sh[4];
-    const int p = blockIdx.x, n = b.pn[p];
-    if (!b.mactive[b.pmol[p]]) return;
-    const int nt = (n + 31) / 32, tile = blockIdx.y, nti = nt * nt;
-    if (tile >= nti) return;
-    double *s = st + b.psp2[p];
-    const double *F = Fp + b.poff[p];
-    double *Xp = X + b.poff[p];
-    const int r0 = (tile / nt) * 32, c0 = (tile % nt) * 32;
-    const double lo = s[0], hi = s[1], sc = 1.0 / (hi - lo);
-    double t1 = 0.0, t2 = 0.0;
-    for (int e = threadIdx.x; e < 1024; e += 256) {
-        const int r = r0 + (e >> 5), c = c0 + (e & 31);
-        if (r < n && c < n) {
-            const double v = ((r == c ? hi : 0.0) - F[(size_t)r * n + c]) * sc;
-            Xp[(size_t)r * n + c] = v;
-            t2 += v * v;
-            if (r == c) t1 += v;
-        }
-    }
-    t1 = block_sum256(t1, sh);
-    t2 = block_sum256(t2, sh);
-    if (threadIdx.x == 0) {
-        s[8 + tile] = t1;
-        s[8 + nti + tile] = t2;
-        if (tile == 0) { s[2] = 1e300; s[3] = 0.0; }
-    }
-}
-__global__ void __launch_bounds__(256) bsp2_step_kernel(BatchDev b, int k, const double *X, double *Xo, double *st) {
-    __shared__ double red[4][16][64];
-    __shared__ double sh[4];
-    const int p = blockIdx.x, n = b.pn[p];
-    if (!b.mactive[b.pmol[p]]) return;
-    const int nt = (n + 31) / 32, tile = blockIdx.y, nti = nt * nt;
-    if (tile >= nti) return;
-    double *s = st + b.psp2[p];
-    const int s0 = k % 3, s1 = (k + 1) % 3;
-    const double age = s[3 + 2 * s0];
-    if (age >= 2.0) {
-        if (tile == 0 && threadIdx.x == 0) { s[2 + 2 * s1] = s[2 + 2 * s0]; s[3 + 2 * s1] = age + 1.0; }
-        return;
-    }
-    const double *P = s + 8;
-    double t1 = 0.0, t2 = 0.0;
-    for (int q = threadIdx.x; q < nti; q += 256) { t1 += P[(2 * s0) * nti + q]; t2 += P[(2 * s0 + 1) * nti + q]; }
-    t1 = block_sum256(t1, sh);
-    t2 = block_sum256(t2, sh);
-    const double nocc = b.pnocc[p], tol = 1e-13 * fmax(1.0, nocc);
-    const double err = fabs(t1 - t2), prev = s[2 + 2 * s0];
-    bool done = age != 0.0;
-    if (!done && k > 2 && (err < tol || (err < 1e-8 && err >= prev))) done = true;
-    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    const double *Xp = X + b.poff[p];
-    double *Xq = Xo + b.poff[p];
-    const int r0 = (tile / nt) * 32, c0 = (tile % nt) * 32;
-    if (!done) mfma_tile(n, n, n, 0, 0, Xp, n, Xp, n, r0, c0, red);
-    double u1 = 0.0, u2 = 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
-        int gr, gc;
-        tile_pos(idx, r0, c0, gr, gc);
-        if (gr < n && gc < n) {
-            const size_t o = (size_t)gr * n + gc;
-            double v;
-            if (done) {
-                v = Xp[o];
-            } else {
-                const double x2 = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
-                v = square ? x2 : 2.0 * Xp[o] - x2;
-            }
-            Xq[o] = v;
-            u2 += v * v;
-            if (gr == gc) u1 += v;
-        }
-    }
-    u1 = block_sum256(u1, sh);
-    u2 = block_sum256(u2, sh);
-    if (threadIdx.x == 0) {
-        s[8 + (2 * s1) * nti + tile] = u1;
-        s[8 + (2 * s1 + 1) * nti + tile] = u2;
-        if (tile == 0) { s[2 + 2 * s1] = err; s[3 + 2 * s1] = done ? age + 1.0 : 0.0; }
     }
 }
 
@@ -440,15 +439,23 @@ BatchEngine::~BatchEngine() {
 }
 
 void BatchEngine::setup() {
+    double tp = now();
+    auto phase = [&](const char *what) {  // verbose: host-side phase times
+        if (!verbose_) return;
+        PF_HIP(hipStreamSynchronize(st_));
+        const double t = now();
+        std::fprintf(stderr, "batch setup: %-28s %8.3f s\n", what, t - tp);
+        tp = t;
+    };
     M_ = (int)mols_.size();
     if (M_ == 0) throw Error("batch: no molecules");
-    const auto lines = read_basis_file(basis_path_);
+    const BasisBlocks blocks = parse_basis_blocks(read_basis_file(basis_path_), basis_path_);
     std::vector<int> shoff(M_ + 1, 0), aoff(M_ + 1, 0);
     std::vector<double> axyz;
     std::vector<int> Z;
     for (int m = 0; m < M_; ++m) {
         const auto &bm = mols_[m];
-        Basis bs = build_basis_lines(lines, basis_path_, bm.species, bm.mol, renorm_);
+        Basis bs = build_basis_blocks(blocks, bm.species, bm.mol, renorm_);
         for (auto &s : bs.shells) shells_.push_back(s);
         shoff[m + 1] = (int)shells_.size();
         aoff[m + 1] = aoff[m] + bm.mol.natoms;
@@ -469,9 +476,9 @@ void BatchEngine::setup() {
     moff_.resize(M_);
     poff_.resize(P_);
     for (int m = 0; m < M_; ++m) { moff_[m] = mtot_; mtot_ += (long long)mn_[m] * mn_[m]; }
-    std::vector<long long> pmoff(P_), pdiis(P_), psp2(P_);
+    std::vector<long long> pmoff(P_), pdiis(P_);
     std::vector<double> poccf(P_), pnocc(P_);
-    long long dtot = 0, stot = 0;
+    long long dtot = 0;
     for (int p = 0; p < P_; ++p) {
         const int m = pmol_[p], n = pn_[p], spin = p - mprob_[m];
         poff_[p] = ptot_;
@@ -479,8 +486,6 @@ void BatchEngine::setup() {
         pmoff[p] = moff_[m];
         pdiis[p] = dtot;
         dtot += (long long)ndiis_ * n * n;
-        psp2[p] = stot;
-        stot += (long long)sp2_size(n);
         poccf[p] = muhf_[m] ? 1.0 : 2.0;
         pnocc[p] = spin == 0 ? mols_[m].nalpha : mols_[m].nbeta;
     }
@@ -490,7 +495,7 @@ void BatchEngine::setup() {
     d_mn_.upload(mn_.data(), M_, st_); d_muhf_.upload(muhf_.data(), M_, st_); d_mprob_.upload(mprob_.data(), M_, st_);
     d_pn_.upload(pn_.data(), P_, st_); d_pmol_.upload(pmol_.data(), P_, st_);
     d_moff_.upload(moff_.data(), M_, st_); d_poff_.upload(poff_.data(), P_, st_);
-    d_pmoff_.upload(pmoff.data(), P_, st_); d_pdiis_.upload(pdiis.data(), P_, st_); d_psp2_.upload(psp2.data(), P_, st_);
+    d_pmoff_.upload(pmoff.data(), P_, st_); d_pdiis_.upload(pdiis.data(), P_, st_);
     d_poccf_.upload(poccf.data(), P_, st_); d_pnocc_.upload(pnocc.data(), P_, st_);
     d_enuc_.upload(enuc_.data(), M_, st_);
     d_mactive_.alloc(M_); d_mprec_.alloc(M_); d_mhead_.alloc(M_); d_mnstore_.alloc(M_); d_muse_.alloc(M_);
@@ -499,31 +504,34 @@ void BatchEngine::setup() {
     d_coef_.alloc((size_t)M_ * ndiis_); d_B_.alloc((size_t)M_ * ndiis_ * ndiis_);
     S_.alloc(mtot_); H_.alloc(mtot_); X_.alloc(mtot_); Dtot_.alloc(mtot_);
     for (auto *b : {&D_, &Dold_, &F_, &Fx_, &Fp_, &W1_, &W2_, &Pr_}) b->alloc(ptot_);
-    sp2_.alloc(stot);
     dF_.alloc(std::max(dtot, 1LL)); dE_.alloc(std::max(dtot, 1LL));
     dev_.M = M_; dev_.P = P_; dev_.nmax = nmax_; dev_.ndiis = ndiis_;
     dev_.mn = d_mn_.p; dev_.muhf = d_muhf_.p; dev_.mprob = d_mprob_.p; dev_.moff = d_moff_.p;
     dev_.pn = d_pn_.p; dev_.pmol = d_pmol_.p; dev_.poccf = d_poccf_.p; dev_.pnocc = d_pnocc_.p;
-    dev_.poff = d_poff_.p; dev_.pmoff = d_pmoff_.p; dev_.pdiis = d_pdiis_.p; dev_.psp2 = d_psp2_.p;
+    dev_.poff = d_poff_.p; dev_.pmoff = d_pmoff_.p; dev_.pdiis = d_pdiis_.p;
     dev_.mactive = d_mactive_.p; dev_.mprec = d_mprec_.p; dev_.mhead = d_mhead_.p; dev_.mnstore = d_mnstore_.p;
     dev_.muse = d_muse_.p; dev_.mcoef = d_coef_.p; dev_.mB = d_B_.p;
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 1, d_mactive_.p);
 
-    // one-electron integrals, one launch per molecule
+    phase("basis sets and offsets");
+    // one-electron integrals of every molecule in one launch
+    d_shells_.upload(shells_.data(), shells_.size(), st_);
     {
-        DevBuf<Shell> dsh;
-        dsh.upload(shells_.data(), shells_.size(), st_);
         DevBuf<double> dxyz, T(mtot_), V(mtot_);
-        DevBuf<int> dZ;
+        DevBuf<int> dZ, dsh, dao;
         dxyz.upload(axyz.data(), axyz.size(), st_);
         dZ.upload(Z.data(), Z.size(), st_);
-        for (int m = 0; m < M_; ++m)
-            launch_onee(dsh.p + shoff[m], shoff[m + 1] - shoff[m], mn_[m], mols_[m].mol.natoms, dxyz.p + 3 * aoff[m],
-                        dZ.p + aoff[m], tab_.p, S_.p + moff_[m], T.p + moff_[m], V.p + moff_[m], st_);
+        dsh.upload(shoff.data(), shoff.size(), st_);
+        dao.upload(aoff.data(), aoff.size(), st_);
+        int max_nsh = 0;
+        for (int m = 0; m < M_; ++m) max_nsh = std::max(max_nsh, shoff[m + 1] - shoff[m]);
+        launch_onee_batch(M_, max_nsh, d_shells_.p, dsh.p, dao.p, d_mn_.p, d_moff_.p, dxyz.p, dZ.p, tab_.p, S_.p, T.p,
+                          V.p, st_);
         dev_axpby(mtot_, 1.0, T.p, 1.0, V.p, H_.p, st_);
         PF_HIP(hipStreamSynchronize(st_));
     }
 
+    phase("device state, 1e integrals");
     // shell pairs per class, all molecules concatenated; pass 1 gives Schwarz bounds
     std::array<std::vector<std::pair<int, int>>, 3> all;
     std::array<std::vector<int>, 3> beg1;
@@ -543,13 +551,14 @@ void BatchEngine::setup() {
     for (int c = 0; c < 3; ++c) {
         PairSet &ps = pairs_[c];
         ps.la = cls_l[c][0]; ps.lb = cls_l[c][1];
-        fill_pairset(ps, shells_, all[c], -1.0, bound_max, st_);
+        fill_pairset(ps, shells_, d_shells_.p, all[c], -1.0, bound_max, false, st_);
         ps.d_Q.alloc(std::max(ps.n, 1));
         if (ps.n) launch_schwarz<double>(ps.la, ps.lb, ps.view64(), tab_.p, ps.d_Q.p, st_);
         Q1[c].resize(ps.n);
         ps.d_Q.download(Q1[c].data(), ps.n, st_);
     }
     PF_HIP(hipStreamSynchronize(st_));
+    phase("pairs pass 1 + Schwarz");
     // pass 2: per molecule keep Q * Qmax(molecule) >= tau, sorted by Q descending
     std::array<std::vector<double>, 3> Q;
     std::array<std::vector<std::pair<int, int>>, 3> kept;
@@ -567,11 +576,17 @@ void BatchEngine::setup() {
             pbeg_[c].push_back((int)kept[c].size());
         }
     }
-    for (int c = 0; c < 3; ++c) fill_pairset(pairs_[c], shells_, kept[c], bound_max, bound_max, st_);
+    for (int c = 0; c < 3; ++c) {
+        fill_pairset(pairs_[c], shells_, d_shells_.p, kept[c], bound_max, bound_max, false, st_);
+        pairs_[c].d_Q.upload(Q[c].data(), Q[c].size(), st_);
+    }
 
+    phase("pairs pass 2");
     // quartet plan per class (bra class x >= ket class y) and the integral store
     PF_HIP(hipStreamSynchronize(st_));
     const double t0 = now();
+    std::array<std::vector<double>, 6> work;  // stored values per bra pair
+    std::array<std::vector<int>, 6> brq;      // per class: bra range of each molecule
     int q = 0;
     for (int x = 0; x < 3; ++x)
         for (int y = 0; y <= x; ++y, ++q) {
@@ -579,6 +594,7 @@ void BatchEngine::setup() {
             const bool same = x == y;
             const int nint = nc(B.la) * nc(B.lb) * nc(K.la) * nc(K.lb);
             std::vector<long long> off(std::max(B.n, 1), 0);
+            work[q].assign(std::max(B.n, 1), 0.0);
             std::vector<int> nk(std::max(B.n, 1), 0), kb(std::max(B.n, 1), 0), br(2 * M_);
             long long tot = 0;
             for (int m = 0; m < M_; ++m) {
@@ -599,6 +615,7 @@ void BatchEngine::setup() {
                     off[i] = tot;
                     tot += (long long)round64(n) * nint;
                     eri_values_ += (double)n * nint;
+                    work[q][i] = (double)round64(n) * nint;
                 }
             }
             s64_[q].alloc(std::max(tot, 1LL));
@@ -607,6 +624,7 @@ void BatchEngine::setup() {
             snket_[q].upload(nk.data(), nk.size(), st_);
             skbase_[q].upload(kb.data(), kb.size(), st_);
             sbr_[q].upload(br.data(), br.size(), st_);
+            brq[q] = br;
             if (B.n) {
                 JKArgs<double> a;
                 a.bra = B.view64(); a.ket = K.view64(); a.same = same; a.tau = schwarz_;
@@ -624,8 +642,56 @@ void BatchEngine::setup() {
         }
     PF_HIP(hipStreamSynchronize(st_));
     t_eri_ = now() - t0;
+    // digest work items: a molecule whose stored integrals exceed ITEM_VALUES
+    // is split into about equal parts (whole bra pairs), so that a few large
+    // molecules spread over many CUs; items go out largest first
+    {
+        constexpr double ITEM_VALUES = 262144.0;
+        std::vector<BatchItem> items;
+        std::vector<double> iwork;
+        std::vector<int> msplit(M_, 0);
+        std::vector<long long> goff(M_);
+        long long gtot = 0;
+        for (int m = 0; m < M_; ++m) {
+            goff[m] = gtot;
+            gtot += 3LL * mn_[m] * (mn_[m] + 1) / 2;
+            double wm = 0.0;
+            for (int c = 0; c < 6; ++c)
+                for (int i = brq[c][2 * m]; i < brq[c][2 * m + 1]; ++i) wm += work[c][i];
+            const int G = std::max(1, (int)std::ceil(wm / ITEM_VALUES));
+            msplit[m] = G > 1;
+            int sq = 0, si = brq[0][2 * m], g = 0;
+            double cum = 0.0, icum = 0.0;
+            for (int c = 0; c < 6 && G > 1; ++c)
+                for (int i = brq[c][2 * m]; i < brq[c][2 * m + 1]; ++i) {
+                    cum += work[c][i];
+                    icum += work[c][i];
+                    if (g < G - 1 && cum >= wm * (g + 1) / G) {
+                        items.push_back({m, sq, si, c, i + 1, 1});
+                        iwork.push_back(icum);
+                        sq = c; si = i + 1; icum = 0.0; ++g;
+                    }
+                }
+            items.push_back({m, sq, si, 5, brq[5][2 * m + 1], G > 1 ? 1 : 0});
+            iwork.push_back(G > 1 ? icum : wm);
+        }
+        std::vector<int> order(items.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return iwork[x] > iwork[y]; });
+        std::vector<BatchItem> sorted;
+        for (int k : order) sorted.push_back(items[k]);
+        nitems_ = (int)sorted.size();
+        items_.upload(sorted.data(), sorted.size(), st_);
+        d_msplit_.upload(msplit.data(), M_, st_);
+        d_gacc_off_.upload(goff.data(), M_, st_);
+        gacc_.alloc(std::max(gtot, 1LL));
+        dev_fill(gacc_.n, 0.0, gacc_.p, st_);
+        dev_.gacc_off = d_gacc_off_.p;
+    }
+    phase("quartet plan + stored ERIs");
     batch_digest_prepare(nmax_);
     setup_orthogonalizer();
+    phase("orthogonalisers");
 }
 
 // X = S^{-1/2} per molecule by the coupled Newton-Schulz iteration on batched
@@ -695,7 +761,7 @@ void BatchEngine::setup_orthogonalizer() {
 
 // D_p = occ_p X P X^T with P the projector of F'_p = X^T F_p X (SP2), and D_alpha + D_beta.
 void BatchEngine::density_step(const double *F) {
-    const dim3 g = pgrid(P_, nmax_), gt(P_, ntile(nmax_)), gr(P_, (nmax_ + 3) / 4);
+    const dim3 g = pgrid(P_, nmax_), gt(P_, ntile(nmax_));
     auto gemm = [&](const double *A, const long long *oA, int ta, const double *B, const long long *oB, int tb,
                     double *C, const double *ascale) {
         BGemm b{A, B, nullptr, C, oA, oB, d_poff_.p, d_poff_.p, ascale, ta, tb, 1.0, 0.0};
@@ -703,21 +769,15 @@ void BatchEngine::density_step(const double *F) {
     };
     gemm(F, d_poff_.p, 0, X_.p, d_pmoff_.p, 0, W1_.p, nullptr);      // F X
     gemm(X_.p, d_pmoff_.p, 1, W1_.p, d_poff_.p, 0, Fp_.p, nullptr);  // X^T F X
-    bgersh_rows_kernel<<<gr, 256, 0, st_>>>(dev_, Fp_.p, sp2_.p);
-    bgersh_bounds_kernel<<<P_, 256, 0, st_>>>(dev_, sp2_.p);
-    bsp2_init_kernel<<<gt, 256, 0, st_>>>(dev_, Fp_.p, sp2_.p, Pr_.p);
-    double *cur = Pr_.p, *nxt = W1_.p;
-    for (int k = 0; k < NSP2; ++k) {
-        bsp2_step_kernel<<<gt, 256, 0, st_>>>(dev_, k, cur, nxt, sp2_.p);
-        std::swap(cur, nxt);
-    }
-    bsym_kernel<<<g, 256, 0, st_>>>(dev_, Pr_.p, W1_.p);                           // NSP2 even: result in Pr
+    bsp2_kernel<<<P_, 256, 0, st_>>>(dev_, Fp_.p, Pr_.p, W2_.p, W1_.p, SP2_MAX_STEPS);  // P into W1
     gemm(X_.p, d_pmoff_.p, 0, W1_.p, d_poff_.p, 0, W2_.p, nullptr);              // X P
     gemm(W2_.p, d_poff_.p, 0, X_.p, d_pmoff_.p, 1, D_.p, d_poccf_.p);            // occ X P X^T
     bdtot_kernel<<<g, 256, 0, st_>>>(dev_, D_.p, Dtot_.p);
 }
 
-void BatchEngine::digest(int waves) { launch_batch_digest(dev_, cls_, D_.p, Dtot_.p, H_.p, F_.p, waves, st_); }
+void BatchEngine::digest(int waves) {
+    launch_batch_digest(dev_, cls_, items_.p, nitems_, d_msplit_.p, D_.p, Dtot_.p, H_.p, F_.p, gacc_.p, waves, st_);
+}
 
 // One SCF iteration for every active molecule (no host decision inside).
 void BatchEngine::iteration(const pf_options &opt) {
@@ -748,6 +808,7 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
     const double t0 = now();
     if (opt.diis > 8) throw Error("batch: DIIS subspace of at most 8");
     ndiis_ = std::max(opt.diis, 2);
+    verbose_ = opt.verbose != 0;
     setup();
     const double t1 = now();
     // state: frozen flags, precision, DIIS rings, energies

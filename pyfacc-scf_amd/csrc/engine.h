@@ -27,11 +27,14 @@ struct PairSet {
     PairView<float> view32() const;
 };
 
-// Fill ps from a list of (first, second) shell indices into sh (device SoA,
-// FP64 and FP32 copies; primitive pairs sorted by bound, significant count kpn
-// against kref, kref < 0 keeps all); bound_max is raised to the largest bound.
-void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const std::vector<std::pair<int, int>> &list,
-                  double kref, double &bound_max, hipStream_t st);
+// Fill ps from a list of (first, second) shell indices into sh (host copy) /
+// d_shells (device copy), built on the device (pairs.hip): SoA fields in FP64
+// (and FP32 when want32), primitive pairs sorted by bound with the significant
+// count kpn against kref (kref < 0 keeps all); bound_max is raised to the
+// largest primitive bound.  Synchronises the stream.
+void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shells,
+                  const std::vector<std::pair<int, int>> &list, double kref, double &bound_max, bool want32,
+                  hipStream_t st);
 
 struct QuartetClass {
     int bra, ket;  // indices into Engine::pairs_

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <numeric>
+#include <thread>
 
 #include "boys.h"
 #include "engine.h"
@@ -98,79 +99,6 @@ void Engine::one_electron(double *S, double *T, double *V) {
     PF_HIP(hipStreamSynchronize(st_));
 }
 
-// Primitive pairs of a shell pair are stored by bound |K|/sqrt(p) descending;
-// kpn counts those with bound * kref >= PRIM_EPS (kref = the largest bound of
-// any pair), the only ones the ERI kernels visit: a primitive quartet is at
-// most K_ab K_cd / sqrt(p+q) times the polynomial factors of the p-type
-// recurrences, so the dropped ones are below 1e-16 in magnitude up to those
-// factors.  kref < 0: keep every primitive (Schwarz pass).  bound_max is
-// raised to the largest bound seen.
-void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const std::vector<std::pair<int, int>> &list,
-                  double kref, double &bound_max, hipStream_t st) {
-    constexpr double PRIM_EPS = 1e-16;
-    ps.n = (int)list.size();
-    ps.npad = round64(std::max(ps.n, 1));
-    int kp = 1;
-    for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
-    ps.kp = kp;
-    std::vector<int> sa(ps.n), sb(ps.n), oa(ps.n), ob(ps.n), kpn(ps.npad, 0);
-    std::vector<double> AB(3 * (size_t)ps.npad, 0.0), prim((size_t)kp * PF_NFIELD * ps.npad, 0.0);
-    for (int u = 0; u < kp; ++u)  // padding entries: unit exponent, zero weight
-        for (int i = 0; i < ps.npad; ++i) {
-            prim[((size_t)u * PF_NFIELD + PF_P) * ps.npad + i] = 1.0;
-            prim[((size_t)u * PF_NFIELD + PF_O2P) * ps.npad + i] = 0.5;
-        }
-    const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
-    std::array<std::array<double, 3>, KMAX * KMAX> order;  // (bound, x, y)
-    for (int i = 0; i < ps.n; ++i) {
-        const Shell &A = sh[list[i].first], &B = sh[list[i].second];
-        sa[i] = list[i].first; sb[i] = list[i].second; oa[i] = A.bf; ob[i] = B.bf;
-        double ab[3], r2 = 0.0;
-        for (int d = 0; d < 3; ++d) { ab[d] = A.xyz[d] - B.xyz[d]; r2 += ab[d] * ab[d]; AB[d * ps.npad + i] = ab[d]; }
-        int no = 0;
-        for (int x = 0; x < A.np; ++x)
-            for (int y = 0; y < B.np; ++y) {
-                const double a = A.exp[x], b = B.exp[y], p = a + b;
-                const double K = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
-                order[no++] = {std::fabs(K) / std::sqrt(p), (double)x, (double)y};
-            }
-        std::stable_sort(order.begin(), order.begin() + no, [](const auto &l, const auto &r) { return l[0] > r[0]; });
-        for (int o = 0; o < no; ++o) {
-            bound_max = std::max(bound_max, order[o][0]);
-            if (kref < 0.0 || order[o][0] * kref >= PRIM_EPS) ++kpn[i];
-        }
-        int u = 0;
-        for (int o = 0; o < no; ++o) {
-            const int x = (int)order[o][1], y = (int)order[o][2];
-            const double a = A.exp[x], b = B.exp[y], p = a + b;
-            double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
-            col[PF_P * ps.npad] = p;
-            for (int d = 0; d < 3; ++d) {
-                const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
-                col[(PF_PX + d) * ps.npad] = P;
-                col[(PF_AX + d) * ps.npad] = P - A.xyz[d];
-            }
-            col[PF_K * ps.npad] = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
-            col[PF_O2P * ps.npad] = 0.5 / p;
-            ++u;
-        }
-        for (; u < kp; ++u) {  // padding primitives contribute zero
-            double *col = prim.data() + (size_t)u * PF_NFIELD * ps.npad + i;
-            col[PF_P * ps.npad] = 1.0;
-            col[PF_K * ps.npad] = 0.0;
-        }
-    }
-    ps.sa = sa; ps.sb = sb;
-    ps.d_sa.upload(sa.data(), ps.n, st); ps.d_sb.upload(sb.data(), ps.n, st);
-    ps.d_oa.upload(oa.data(), ps.n, st); ps.d_ob.upload(ob.data(), ps.n, st);
-    ps.d_kpn.upload(kpn.data(), kpn.size(), st);
-    ps.d_AB.upload(AB.data(), AB.size(), st);
-    ps.d_prim.upload(prim.data(), prim.size(), st);
-    std::vector<float> fAB(AB.begin(), AB.end()), fprim(prim.begin(), prim.end());
-    ps.f_AB.upload(fAB.data(), fAB.size(), st);
-    ps.f_prim.upload(fprim.data(), fprim.size(), st);
-}
-
 // Shell pairs: class 0 = (s,s), 1 = (p,s), 2 = (p,p); the first shell of a
 // pair has the higher angular momentum (ties: the higher shell index).
 void Engine::setup_pairs() {
@@ -186,7 +114,7 @@ void Engine::setup_pairs() {
         }
     double bound_max = 0.0;
     auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list, double kref) {
-        fill_pairset(ps, sh, list, kref, bound_max, st_);
+        fill_pairset(ps, sh, d_shells_.p, list, kref, bound_max, true, st_);
     };
     const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
     // pass 1: all pairs, Schwarz bounds

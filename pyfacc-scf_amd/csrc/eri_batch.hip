@@ -23,12 +23,11 @@ struct BatchClasses {
 };
 
 template <int LA, int LB, int LC, int LD, int W>
-__device__ __forceinline__ void digest_class(const BatchClass &c, int m, const DensView<double> &dv, bool f32,
-                                             int nk, double *lj, double *lk0, double *lk1) {
+__device__ __forceinline__ void digest_class(const BatchClass &c, int b0, int b1, const DensView<double> &dv,
+                                             bool f32, int nk, double *lj, double *lk0, double *lk1) {
     using E = Eri<LA, LB, LC, LD, double>;
     constexpr int NA = E::NA, NB = E::NB, NINT = E::NINT;
     const int lane = threadIdx.x & 63;
-    const int b0 = c.brange[2 * m], b1 = c.brange[2 * m + 1];
     for (int i = b0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); i < b1; i += W) {
         const int a = c.sa[i], b = c.sb[i], oa = c.oa[i], ob = c.ob[i];
         const int nket = c.nket[i], kb = c.kbase[i];
@@ -74,11 +73,17 @@ __device__ __forceinline__ void digest_class(const BatchClass &c, int m, const D
     }
 }
 
+// Work item: molecule m's bra pairs from (class q0, pair i0) to (class q1,
+// pair i1) exclusive, in class order.  A molecule with one item writes its
+// Fock matrices directly; one split into several items adds its LDS
+// triangles to gacc (device atomics) and bfock_from_acc forms F.
 template <int W>
-__global__ void __launch_bounds__(64 * W) batch_digest_kernel(BatchDev b, BatchClasses cls, const double *D,
-                                                               const double *Dtot, const double *H, double *F) {
+__global__ void __launch_bounds__(64 * W) batch_digest_kernel(BatchDev b, BatchClasses cls, const BatchItem *items,
+                                                               const double *D, const double *Dtot, const double *H,
+                                                               double *F, double *gacc) {
     extern __shared__ double lds[];
-    const int m = blockIdx.x;
+    const BatchItem it = items[blockIdx.x];
+    const int m = it.m;
     if (!b.mactive[m]) return;
     const int n = b.mn[m], uhf = b.muhf[m], nk = uhf ? 2 : 1;
     const int nt = n * (n + 1) / 2;
@@ -89,14 +94,26 @@ __global__ void __launch_bounds__(64 * W) batch_digest_kernel(BatchDev b, BatchC
     const double *D0 = D + b.poff[p0], *D1 = uhf ? D + b.poff[p0 + 1] : D0;
     const DensView<double> dv{uhf ? Dtot + b.moff[m] : D0, D0, D1, n};
     const bool f32 = b.mprec[m] == 0;
+    auto range = [&](int q, int &r0, int &r1) {
+        r0 = q == it.q0 ? it.i0 : cls.c[q].brange[2 * m];
+        r1 = q == it.q1 ? it.i1 : cls.c[q].brange[2 * m + 1];
+        if (q < it.q0 || q > it.q1) r1 = r0;
+    };
+    int r0, r1;
     // classes in the engine's order: (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)
-    digest_class<0, 0, 0, 0, W>(cls.c[0], m, dv, f32, nk, lj, lk0, lk1);
-    digest_class<1, 0, 0, 0, W>(cls.c[1], m, dv, f32, nk, lj, lk0, lk1);
-    digest_class<1, 0, 1, 0, W>(cls.c[2], m, dv, f32, nk, lj, lk0, lk1);
-    digest_class<1, 1, 0, 0, W>(cls.c[3], m, dv, f32, nk, lj, lk0, lk1);
-    digest_class<1, 1, 1, 0, W>(cls.c[4], m, dv, f32, nk, lj, lk0, lk1);
-    digest_class<1, 1, 1, 1, W>(cls.c[5], m, dv, f32, nk, lj, lk0, lk1);
+    range(0, r0, r1); digest_class<0, 0, 0, 0, W>(cls.c[0], r0, r1, dv, f32, nk, lj, lk0, lk1);
+    range(1, r0, r1); digest_class<1, 0, 0, 0, W>(cls.c[1], r0, r1, dv, f32, nk, lj, lk0, lk1);
+    range(2, r0, r1); digest_class<1, 0, 1, 0, W>(cls.c[2], r0, r1, dv, f32, nk, lj, lk0, lk1);
+    range(3, r0, r1); digest_class<1, 1, 0, 0, W>(cls.c[3], r0, r1, dv, f32, nk, lj, lk0, lk1);
+    range(4, r0, r1); digest_class<1, 1, 1, 0, W>(cls.c[4], r0, r1, dv, f32, nk, lj, lk0, lk1);
+    range(5, r0, r1); digest_class<1, 1, 1, 1, W>(cls.c[5], r0, r1, dv, f32, nk, lj, lk0, lk1);
     __syncthreads();
+    if (it.split) {
+        double *g = gacc + b.gacc_off[m];
+        for (int e = threadIdx.x; e < (1 + nk) * nt; e += 64 * W)
+            if (lds[e] != 0.0) atomicAdd(&g[e], lds[e]);
+        return;
+    }
     const double *Hm = H + b.moff[m];
     double *F0 = F + b.poff[p0], *F1 = uhf ? F + b.poff[p0 + 1] : nullptr;
     for (int e = threadIdx.x; e < n * n; e += 64 * W) {
@@ -112,18 +129,45 @@ __global__ void __launch_bounds__(64 * W) batch_digest_kernel(BatchDev b, BatchC
     }
 }
 
+// F from the summed triangles of split molecules, then the sums are cleared.
+__global__ void __launch_bounds__(256) bfock_from_acc_kernel(BatchDev b, const int *msplit, const double *H,
+                                                             double *F, double *gacc) {
+    const int m = blockIdx.x;
+    if (!b.mactive[m] || !msplit[m]) return;
+    const int n = b.mn[m], uhf = b.muhf[m], nt = n * (n + 1) / 2, p0 = b.mprob[m];
+    double *g = gacc + b.gacc_off[m];
+    const double *lj = g, *lk0 = g + nt, *lk1 = g + 2 * nt;
+    const double *Hm = H + b.moff[m];
+    double *F0 = F + b.poff[p0], *F1 = uhf ? F + b.poff[p0 + 1] : nullptr;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int r = e / n, c = e - r * n, t = tri(r, c);
+        const double f = r == c ? 2.0 : 1.0;
+        const double jv = f * lj[t];
+        if (!uhf) {
+            F0[e] = Hm[e] + jv - 0.5 * f * lk0[t];
+        } else {
+            F0[e] = Hm[e] + jv - f * lk0[t];
+            F1[e] = Hm[e] + jv - f * lk1[t];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (uhf ? 3 : 2) * nt; e += 256) g[e] = 0.0;
+}
+
 }  // namespace
 
 size_t batch_digest_lds(int nmax) { return sizeof(double) * 3 * (size_t)nmax * (nmax + 1) / 2; }
 
-void launch_batch_digest(const BatchDev &b, const std::array<BatchClass, 6> &cls, const double *D,
-                         const double *Dtot, const double *H, double *F, int waves, hipStream_t st) {
+void launch_batch_digest(const BatchDev &b, const std::array<BatchClass, 6> &cls, const BatchItem *items,
+                         int nitems, const int *msplit, const double *D, const double *Dtot, const double *H,
+                         double *F, double *gacc, int waves, hipStream_t st) {
     BatchClasses c;
     for (int q = 0; q < 6; ++q) c.c[q] = cls[q];
     const size_t lds = batch_digest_lds(b.nmax);
-    if (waves == 4) batch_digest_kernel<4><<<b.M, 256, lds, st>>>(b, c, D, Dtot, H, F);
-    else if (waves == 8) batch_digest_kernel<8><<<b.M, 512, lds, st>>>(b, c, D, Dtot, H, F);
-    else throw Error("batch digest: 4 or 8 wavefronts per molecule");
+    if (waves == 4) batch_digest_kernel<4><<<nitems, 256, lds, st>>>(b, c, items, D, Dtot, H, F, gacc);
+    else if (waves == 8) batch_digest_kernel<8><<<nitems, 512, lds, st>>>(b, c, items, D, Dtot, H, F, gacc);
+    else throw Error("batch digest: 4 or 8 wavefronts per item");
+    bfock_from_acc_kernel<<<b.M, 256, 0, st>>>(b, msplit, H, F, gacc);
     PF_HIP(hipGetLastError());
 }
 

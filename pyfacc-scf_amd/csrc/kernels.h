@@ -9,6 +9,11 @@ namespace pf {
 // ---------------------------------------------------------------- 1e
 void launch_onee(const Shell *d_shells, int nsh, int nbf, int natoms, const double *d_axyz, const int *d_Z,
                  const double *d_tab, double *S, double *T, double *V, hipStream_t st);
+// Many molecules at once (device arrays: shells from shoff[m], atoms from
+// aoff[m], both with nmol + 1 entries; matrices of nbf mnbf[m] at moff[m]).
+void launch_onee_batch(int nmol, int max_nsh, const Shell *d_shells, const int *shoff, const int *aoff,
+                       const int *mnbf, const long long *moff, const double *d_axyz, const int *d_Z,
+                       const double *d_tab, double *S, double *T, double *V, hipStream_t st);
 
 // ---------------------------------------------------------------- pairs
 // Per primitive pair data, in this field order, stored structure-of-arrays.

@@ -139,20 +139,11 @@ __device__ void onee_block(const Shell &A, const Shell &B, int natoms, const dou
     }
 }
 
-__global__ void __launch_bounds__(256) onee_kernel(const Shell *__restrict__ shells, int nsh, int nbf, int natoms,
-                                                   const double *__restrict__ axyz, const int *__restrict__ Z,
-                                                   const double *__restrict__ gtab, double *Smat, double *Tmat,
-                                                   double *Vmat) {
-    // orders 0 .. 2 + 6 (enough for LA + LB <= 2) staged in LDS, stride ONEE_NORD
-    extern __shared__ double lds_tab[];
-    for (int k = threadIdx.x; k < BOYS_NGRID * ONEE_NORD; k += blockDim.x) {
-        const int g = k / ONEE_NORD, m = k - g * ONEE_NORD;
-        lds_tab[k] = gtab[g * (BOYS_MMAX + 1) + m];
-    }
-    __syncthreads();
-    const int npair = nsh * (nsh + 1) / 2;
-    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid >= npair) return;
+// One shell pair (a >= b, pair index pid) of one molecule: S, T, V blocks
+// written to both triangles.
+__device__ __forceinline__ void onee_pair(const Shell *__restrict__ shells, int pid, int nbf, int natoms,
+                                          const double *__restrict__ axyz, const int *__restrict__ Z,
+                                          const double *lds_tab, double *Smat, double *Tmat, double *Vmat) {
     // pair index -> (a, b), a >= b
     int a = int((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
     while (a * (a + 1) / 2 > pid) --a;
@@ -174,6 +165,55 @@ __global__ void __launch_bounds__(256) onee_kernel(const Shell *__restrict__ she
             Tmat[mu * nbf + nu] = t[k]; Tmat[nu * nbf + mu] = t[k];
             Vmat[mu * nbf + nu] = v[k]; Vmat[nu * nbf + mu] = v[k];
         }
+}
+
+__device__ __forceinline__ void onee_stage_table(const double *__restrict__ gtab, double *lds_tab) {
+    // orders 0 .. 2 + 6 (enough for LA + LB <= 2) staged in LDS, stride ONEE_NORD
+    for (int k = threadIdx.x; k < BOYS_NGRID * ONEE_NORD; k += blockDim.x) {
+        const int g = k / ONEE_NORD, m = k - g * ONEE_NORD;
+        lds_tab[k] = gtab[g * (BOYS_MMAX + 1) + m];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) onee_kernel(const Shell *__restrict__ shells, int nsh, int nbf, int natoms,
+                                                   const double *__restrict__ axyz, const int *__restrict__ Z,
+                                                   const double *__restrict__ gtab, double *Smat, double *Tmat,
+                                                   double *Vmat) {
+    extern __shared__ double lds_tab[];
+    onee_stage_table(gtab, lds_tab);
+    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= nsh * (nsh + 1) / 2) return;
+    onee_pair(shells, pid, nbf, natoms, axyz, Z, lds_tab, Smat, Tmat, Vmat);
+}
+
+// Many molecules in one launch: blockIdx.y = molecule m, whose shells start at
+// shoff[m], atoms at aoff[m] and matrices at moff[m].
+__global__ void __launch_bounds__(256) onee_batch_kernel(const Shell *__restrict__ shells, const int *shoff,
+                                                         const int *aoff, const int *mnbf, const long long *moff,
+                                                         const double *__restrict__ axyz, const int *__restrict__ Z,
+                                                         const double *__restrict__ gtab, double *Smat, double *Tmat,
+                                                         double *Vmat) {
+    extern __shared__ double lds_tab[];
+    const int m = blockIdx.y;
+    const int nsh = shoff[m + 1] - shoff[m];
+    if ((int)(blockIdx.x * blockDim.x) >= nsh * (nsh + 1) / 2) return;
+    onee_stage_table(gtab, lds_tab);
+    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= nsh * (nsh + 1) / 2) return;
+    const size_t o = moff[m];
+    onee_pair(shells + shoff[m], pid, mnbf[m], aoff[m + 1] - aoff[m], axyz + 3 * (size_t)aoff[m], Z + aoff[m], lds_tab,
+              Smat + o, Tmat + o, Vmat + o);
+}
+
+void launch_onee_batch(int nmol, int max_nsh, const Shell *d_shells, const int *shoff, const int *aoff,
+                       const int *mnbf, const long long *moff, const double *d_axyz, const int *d_Z,
+                       const double *d_tab, double *S, double *T, double *V, hipStream_t st) {
+    const int npair = max_nsh * (max_nsh + 1) / 2;
+    const dim3 grid((npair + 255) / 256, nmol);
+    const size_t lds = sizeof(double) * BOYS_NGRID * ONEE_NORD;
+    onee_batch_kernel<<<grid, 256, lds, st>>>(d_shells, shoff, aoff, mnbf, moff, d_axyz, d_Z, d_tab, S, T, V);
+    PF_HIP(hipGetLastError());
 }
 
 void launch_onee(const Shell *d_shells, int nsh, int nbf, int natoms, const double *d_axyz, const int *d_Z,

@@ -1,0 +1,160 @@
+// Shell-pair structure-of-arrays built on the device, one thread per pair.
+//
+// For pair i = (A, B) the thread computes every primitive pair's
+// K = sqrt(2) pi^{5/4} c_a c_b exp(-ab/p |AB|^2) / p and bound |K|/sqrt(p),
+// orders the primitive pairs by bound descending (stable), counts those whose
+// bound times kref reaches PRIM_EPS (kpn: the only ones the ERI kernels
+// visit; a primitive quartet is at most K_ab K_cd / sqrt(p+q) times the
+// polynomial factors of the p-type recurrences, so the dropped ones are below
+// 1e-16 up to those factors; kref < 0 keeps every primitive), and writes the
+// fields P, P - A, K, 1/2p with stride npad.  Padding pairs and primitives get
+// a unit exponent and zero weight.
+#include "engine.h"
+
+namespace pf {
+
+namespace {
+
+constexpr double PRIM_EPS = 1e-16;
+
+__global__ void __launch_bounds__(256) build_pairs_kernel(const Shell *__restrict__ sh, const int *__restrict__ list,
+                                                          int n, int npad, int kp, double kref, double *prim,
+                                                          double *AB, int *sa, int *sb, int *oa, int *ob, int *kpn,
+                                                          double *bmax) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npad) return;
+    auto pad_prim = [&](int u) {
+        double *col = prim + (size_t)u * PF_NFIELD * npad + i;
+        for (int f = 0; f < PF_NFIELD; ++f) col[f * npad] = 0.0;
+        col[PF_P * npad] = 1.0;
+        col[PF_O2P * npad] = 0.5;
+    };
+    if (i >= n) {
+        for (int u = 0; u < kp; ++u) pad_prim(u);
+        for (int d = 0; d < 3; ++d) AB[d * npad + i] = 0.0;
+        kpn[i] = 0;
+        return;
+    }
+    const double kfac = sqrt(2.0) * pow(M_PI, 1.25);
+    const int ia = list[2 * i], ib = list[2 * i + 1];
+    const Shell &A = sh[ia], &B = sh[ib];
+    sa[i] = ia;
+    sb[i] = ib;
+    oa[i] = A.bf;
+    ob[i] = B.bf;
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double ab = A.xyz[d] - B.xyz[d];
+        AB[d * npad + i] = ab;
+        r2 += ab * ab;
+    }
+    double bnd[KMAX * KMAX];
+    int ord[KMAX * KMAX];
+    int no = 0;
+    for (int x = 0; x < A.np; ++x)
+        for (int y = 0; y < B.np; ++y) {
+            const double a = A.exp[x], b = B.exp[y], p = a + b;
+            const double K = kfac * A.coef[x] * B.coef[y] * exp(-a * b / p * r2) / p;
+            bnd[no] = fabs(K) / sqrt(p);
+            ord[no] = x * KMAX + y;
+            ++no;
+        }
+    // stable selection sort, bound descending
+    for (int s = 0; s < no; ++s) {
+        int best = s;
+        for (int t = s + 1; t < no; ++t)
+            if (bnd[t] > bnd[best]) best = t;
+        const double bb = bnd[best];
+        const int ob_ = ord[best];
+        for (int t = best; t > s; --t) { bnd[t] = bnd[t - 1]; ord[t] = ord[t - 1]; }
+        bnd[s] = bb;
+        ord[s] = ob_;
+    }
+    int cnt = 0;
+    for (int u = 0; u < no; ++u) {
+        if (kref < 0.0 || bnd[u] * kref >= PRIM_EPS) ++cnt;
+        const int x = ord[u] / KMAX, y = ord[u] % KMAX;
+        const double a = A.exp[x], b = B.exp[y], p = a + b;
+        double *col = prim + (size_t)u * PF_NFIELD * npad + i;
+        col[PF_P * npad] = p;
+        for (int d = 0; d < 3; ++d) {
+            const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
+            col[(PF_PX + d) * npad] = P;
+            col[(PF_AX + d) * npad] = P - A.xyz[d];
+        }
+        col[PF_K * npad] = kfac * A.coef[x] * B.coef[y] * exp(-a * b / p * r2) / p;
+        col[PF_O2P * npad] = 0.5 / p;
+    }
+    for (int u = no; u < kp; ++u) pad_prim(u);
+    kpn[i] = cnt;
+    bmax[i] = no ? bnd[0] : 0.0;
+}
+
+__global__ void to_f32_kernel(size_t n, const double *x, float *y) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = float(x[i]);
+}
+
+__global__ void __launch_bounds__(1024) max_reduce_kernel(int n, const double *x, double *out) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, x[i]);
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < 16; ++w) m = fmax(m, sh[w]);
+        *out = m;
+    }
+}
+
+int round64(int n) { return ((n + 63) / 64) * 64; }
+
+}  // namespace
+
+void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shells,
+                  const std::vector<std::pair<int, int>> &list, double kref, double &bound_max, bool want32,
+                  hipStream_t st) {
+    ps.n = (int)list.size();
+    ps.npad = round64(std::max(ps.n, 1));
+    int kp = 1;
+    for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
+    ps.kp = kp;
+    ps.sa.resize(ps.n);
+    ps.sb.resize(ps.n);
+    std::vector<int> flat(2 * (size_t)std::max(ps.n, 1), 0);
+    for (int i = 0; i < ps.n; ++i) {
+        ps.sa[i] = list[i].first;
+        ps.sb[i] = list[i].second;
+        flat[2 * i] = list[i].first;
+        flat[2 * i + 1] = list[i].second;
+    }
+    DevBuf<int> dlist;
+    dlist.upload(flat.data(), flat.size(), st);
+    const size_t nprim = (size_t)kp * PF_NFIELD * ps.npad;
+    ps.d_sa.alloc(ps.npad); ps.d_sb.alloc(ps.npad); ps.d_oa.alloc(ps.npad); ps.d_ob.alloc(ps.npad);
+    ps.d_kpn.alloc(ps.npad);
+    ps.d_AB.alloc(3 * (size_t)ps.npad);
+    ps.d_prim.alloc(nprim);
+    DevBuf<double> bmax(std::max(ps.n, 1)), red(1);
+    build_pairs_kernel<<<(ps.npad + 255) / 256, 256, 0, st>>>(d_shells, dlist.p, ps.n, ps.npad, kp, kref,
+                                                              ps.d_prim.p, ps.d_AB.p, ps.d_sa.p, ps.d_sb.p,
+                                                              ps.d_oa.p, ps.d_ob.p, ps.d_kpn.p, bmax.p);
+    PF_HIP(hipGetLastError());
+    if (want32) {
+        ps.f_AB.alloc(3 * (size_t)ps.npad);
+        ps.f_prim.alloc(nprim);
+        to_f32_kernel<<<(unsigned)((3 * (size_t)ps.npad + 255) / 256), 256, 0, st>>>(3 * (size_t)ps.npad, ps.d_AB.p,
+                                                                                     ps.f_AB.p);
+        to_f32_kernel<<<(unsigned)((nprim + 255) / 256), 256, 0, st>>>(nprim, ps.d_prim.p, ps.f_prim.p);
+    }
+    double m = 0.0;
+    if (ps.n) {
+        max_reduce_kernel<<<1, 1024, 0, st>>>(ps.n, bmax.p, red.p);
+        PF_HIP(hipMemcpyAsync(&m, red.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    PF_HIP(hipStreamSynchronize(st));
+    bound_max = std::max(bound_max, m);
+}
+
+}  // namespace pf

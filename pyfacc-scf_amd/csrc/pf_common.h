@@ -58,10 +58,28 @@ struct Basis {
 // Basis construction (basis.cpp).
 Basis build_basis(const std::string &path, const std::vector<std::string> &species, const Molecule &mol,
                   bool renormalise);
-// The same from the file's lines, read once by read_basis_file (batched use).
+// The same in steps, so that a batch parses the file once: the file's lines,
+// its element blocks (in file order, shell lines with their rows), and the
+// basis of one molecule from the blocks.
+struct BasisBlocks {
+    struct ShellLine {
+        std::string kind;
+        int np = 0;
+        std::vector<std::vector<double>> rows;
+    };
+    struct Block {
+        std::string label;
+        std::vector<ShellLine> shells;
+    };
+    std::string path;
+    std::vector<Block> blocks;
+};
 std::vector<std::string> read_basis_file(const std::string &path);
+BasisBlocks parse_basis_blocks(const std::vector<std::string> &lines, const std::string &path);
 Basis build_basis_lines(const std::vector<std::string> &lines, const std::string &path,
                         const std::vector<std::string> &species, const Molecule &mol, bool renormalise);
+Basis build_basis_blocks(const BasisBlocks &blocks, const std::vector<std::string> &species, const Molecule &mol,
+                         bool renormalise);
 std::string resolve_basis_path(const std::string &name);
 double nuclear_repulsion(const Molecule &mol);
 
