@@ -115,6 +115,11 @@ class Engine {
     DevBuf<double> Dj64_;
     DevBuf<unsigned long long> count_;
     std::array<DevBuf<int>, 6> plan_nket_, plan_off_, plan_work_;  // direct-mode work plans per class
+    std::array<DevBuf<long long>, 6> plan_joff_;
+    std::array<DevBuf<int>, 6> plan_jhi_;
+    std::array<DevBuf<double>, 6> jbuf_;  // per class: J'_cd of each evaluated quartet
+    std::array<long long, 6> jcap_{};
+    static constexpr long long JBUF_MAX_VALUES = 1LL << 27;  // per class (1 GiB), beyond: device atomics
     // stored integrals
     bool store_ready_ = false;
     std::array<DevBuf<double>, 6> store_;

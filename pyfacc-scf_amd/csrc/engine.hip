@@ -166,8 +166,16 @@ void Engine::setup_pairs() {
     Dsh64_.alloc((size_t)nshell() * nshell()); dmax64_.alloc(1);
     count_.alloc(12);
     for (size_t q = 0; q < classes_.size(); ++q) {  // direct-mode work plans
-        const int nb = std::max(pairs_[classes_[q].bra].n, 1);
-        plan_nket_[q].alloc(nb); plan_off_[q].alloc(nb + 1); plan_work_[q].alloc(2);
+        const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
+        const int nb = std::max(B.n, 1);
+        plan_nket_[q].alloc(nb); plan_off_[q].alloc(nb + 1); plan_work_[q].alloc(3); plan_joff_[q].alloc(nb);
+        // J'_cd buffer: every (bra, ket) slot the largest possible prefixes can reach, capped
+        const bool same = classes_[q].bra == classes_[q].ket;
+        const long long full = same ? (long long)B.n * (B.n + 1) / 2 : (long long)B.n * K.n;
+        const long long ncd = (long long)ncart(K.la) * ncart(K.lb);
+        jcap_[q] = std::min<long long>(full, JBUF_MAX_VALUES / ncd);
+        jbuf_[q].alloc(std::max<long long>(jcap_[q] * ncd, 1));
+        plan_jhi_[q].alloc(std::max(K.n, 1));
     }
     PF_HIP(hipStreamSynchronize(st_));
     pairs_ready_ = true;
@@ -274,6 +282,7 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         a.count = count_.p + classes_[q].slot;
         a.pcount = count_.p + 6 + classes_[q].slot;
         a.nket = plan_nket_[q].p; a.item_off = plan_off_[q].p; a.work = plan_work_[q].p;
+        a.joff = plan_joff_[q].p; a.jbuf = jbuf_[q].p; a.jcap = jcap_[q]; a.jhi = plan_jhi_[q].p;
         if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }

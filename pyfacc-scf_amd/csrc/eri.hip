@@ -23,30 +23,43 @@ extern template void launch_nk_mode<1, 0, 1, 1, float>(int, int, const JKArgs<fl
 extern template void launch_nk_mode<1, 1, 1, 1, double>(int, int, const JKArgs<double> &, hipStream_t);
 extern template void launch_nk_mode<1, 1, 1, 1, float>(int, int, const JKArgs<float> &, hipStream_t);
 
-__global__ void __launch_bounds__(1024) jk_scan_kernel(int n, int *item_off, int *work) {
+__global__ void __launch_bounds__(1024) jk_scan_kernel(int n, int *item_off, const int *nket, long long *joff,
+                                                       long long jcap, int *work) {
     __shared__ int part[1024];
+    __shared__ long long jpart[1024];
     const int t = threadIdx.x, per = (n + 1023) / 1024;
     const int b0 = min(n, t * per), b1 = min(n, b0 + per);
     int s = 0;
-    for (int i = b0; i < b1; ++i) s += item_off[i];
+    long long js = 0;
+    for (int i = b0; i < b1; ++i) {
+        s += item_off[i];
+        js += nket[i];
+    }
     part[t] = s;
+    jpart[t] = js;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the per-thread sums
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans of the per-thread sums
         const int v = t >= o ? part[t - o] : 0;
+        const long long jv = t >= o ? jpart[t - o] : 0;
         __syncthreads();
         part[t] += v;
+        jpart[t] += jv;
         __syncthreads();
     }
     int off = part[t] - s;
+    long long joff_ = jpart[t] - js;
     for (int i = b0; i < b1; ++i) {
         const int c = item_off[i];
         item_off[i] = off;
         off += c;
+        joff[i] = joff_;
+        joff_ += nket[i];
     }
     if (t == 1023) {
         item_off[n] = part[1023];
         work[0] = 0;
         work[1] = part[1023];
+        work[2] = jpart[1023] <= jcap;  // the J'_cd buffer holds this build's ket lists
     }
 }
 

@@ -347,6 +347,8 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
     int *list = lds_list[wave];
     const bool compact = g.Dsh != nullptr;
     const int total = g.work[1];
+    constexpr int NCD = E::NC * E::ND;
+    const bool jstore = g.jbuf != nullptr && g.work[2] != 0;
     unsigned long long counted = 0, pcounted = 0;
     __syncthreads();
     while (true) {
@@ -392,6 +394,11 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
                         dq = max(dq, max(max(g.Dsh[b * ns + d], g.Dsh[b * ns + c]),
                                          max(g.Dsh[a * ns + d], g.Dsh[a * ns + c])));
                         keep = Qi * K.Q[jj] * dq >= g.tau;
+                        if (!keep && jstore) {  // a screened ket adds nothing to its J'_cd column
+                            double *jb = g.jbuf + (g.joff[i] + jj) * NCD;
+#pragma unroll
+                            for (int q = 0; q < NCD; ++q) jb[q] = 0.0;
+                        }
                     }
                     const unsigned long long m = __ballot(keep);
                     if (keep) {
@@ -423,7 +430,8 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
             R eri[NINT];
             eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
-            const auto jadd = [&](int row, int col, R v) { atomicAdd(&g.J[(size_t)row * nbf + col], double(v)); };
+            double jcd[NCD];  // this quartet's J'_cd block
+            const auto jadd = [&](int row, int col, R v) { jcd[(row - oc) * E::ND + (col - od)] = double(v); };
             if (g.krows) {
                 digest<LA, LB, LC, LD, R, NK>(dens_view(g), eri, oa, ob, oc, od, deg, dab, jab, jadd,
                                               [&](int kk, int r, int col, R v) {
@@ -436,6 +444,15 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
                                                   const int row = r < NA ? oa + r : ob + r - NA;
                                                   atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
                                               });
+            }
+            if (jstore) {
+                double *jb = g.jbuf + (g.joff[i] + j) * NCD;
+#pragma unroll
+                for (int q = 0; q < NCD; ++q) jb[q] = jcd[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NCD; ++q)
+                    atomicAdd(&g.J[(size_t)(oc + q / E::ND) * nbf + od + q % E::ND], jcd[q]);
             }
         }
 #pragma unroll
@@ -524,13 +541,50 @@ __global__ void __launch_bounds__(256) jk_count_kernel(JKArgs<R> g) {
 
 // Step 2 (one workgroup): exclusive prefix sum of the item counts in place,
 // item_off[n] = total, and the queue head reset (work[0] = 0, work[1] = total).
-__global__ void __launch_bounds__(1024) jk_scan_kernel(int n, int *item_off, int *work);
+__global__ void __launch_bounds__(1024) jk_scan_kernel(int n, int *item_off, const int *nket, long long *joff,
+                                                       long long jcap, int *work);
+
+// Step 4 (after the direct kernel, when the J'_cd buffer was used): column
+// sums.  The bras whose prefix holds ket j form a contiguous range [lo, hi):
+// prefixes shrink as Q_bra falls (same-class lists start at bra = ket), so
+// jk_jrange_kernel finds hi by binary search.  jcd_reduce_kernel then takes
+// JRED_TILE bras per blockIdx.y and one ket-pair component per thread; the
+// tile's sum goes to J' with one device atomic (one per component and tile
+// instead of one per quartet).
+template <typename R>
+__global__ void __launch_bounds__(256) jk_jrange_kernel(JKArgs<R> g, int *jhi) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= g.ket.n || !g.work[2]) return;
+    int lo = g.same ? j : 0, hi = g.bra.n;  // first bra with nket <= j
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (g.nket[mid] > j) lo = mid + 1;
+        else hi = mid;
+    }
+    jhi[j] = lo;
+}
+constexpr int JRED_TILE = 256;
+template <int NC, int ND, typename R>
+__global__ void __launch_bounds__(256) jcd_reduce_kernel(JKArgs<R> g, const int *jhi) {
+    constexpr int NCD = NC * ND;
+    if (!g.work[2]) return;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int j = (int)(t / NCD), k = (int)(t - (long long)j * NCD);
+    if (j >= g.ket.n) return;
+    const int i0 = max((int)blockIdx.y * JRED_TILE, g.same ? j : 0);
+    const int i1 = min(jhi[j], (int)(blockIdx.y + 1) * JRED_TILE);
+    double s = 0.0;
+#pragma unroll 8
+    for (int i = i0; i < i1; ++i) s += g.jbuf[(g.joff[i] + j) * NCD + k];
+    if (s != 0.0) atomicAdd(&g.J[(size_t)(g.ket.oa[j] + k / ND) * g.nbf + g.ket.ob[j] + k % ND], s);
+}
 
 template <int LA, int LB, int LC, int LD, typename R, int NK>
 void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     constexpr int W = jk_waves<LA, LB, LC, LD>();
     constexpr int NA = nc(LA), NB = nc(LB);
-    if (!args.work || !args.item_off || !args.nket) throw Error("jk: direct mode needs its work-plan buffers");
+    if (!args.work || !args.item_off || !args.nket || !args.joff)
+        throw Error("jk: direct mode needs its work-plan buffers");
     JKArgs<R> a = args;
     a.chunk = jk_chunk<LA, LB, LC, LD>();
     const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK>;
@@ -549,8 +603,15 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * W, dyn));
     const int grid = std::max(1, per) * std::max(1, ncu);  // persistent: as many workgroups as fit at once
     jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
-    jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.work);
+    jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);
     kern<<<grid, 64 * W, dyn, st>>>(a, a.bra.prim, a.item_off);
+    if (a.jbuf) {
+        constexpr int NCD = nc(LC) * nc(LD);
+        const long long nt = (long long)a.ket.n * NCD;
+        jk_jrange_kernel<R><<<(a.ket.n + 255) / 256, 256, 0, st>>>(a, a.jhi);
+        const dim3 grid((unsigned)((nt + 255) / 256), (a.bra.n + JRED_TILE - 1) / JRED_TILE);
+        jcd_reduce_kernel<nc(LC), nc(LD), R><<<grid, 256, 0, st>>>(a, a.jhi);
+    }
 }
 
 template <int LA, int LB, int LC, int LD, typename R>

@@ -63,9 +63,17 @@ struct JKArgs {
     unsigned long long *count = nullptr;
     unsigned long long *pcount = nullptr;  // primitive quartets evaluated, may be nullptr
     // direct-mode work plan (device): per bra ket-prefix length [bra.n],
-    // first work item [bra.n + 1], queue head and item total [2]; kets per item
+    // first work item [bra.n + 1], queue head, item total and J'_cd-buffer
+    // flag [3]; kets per item
     int *nket = nullptr, *item_off = nullptr, *work = nullptr;
     int chunk = 0;
+    // J'_cd of every evaluated quartet, stored per (bra, ket of its prefix) at
+    // (joff[bra] + ket) * NC*ND and summed per ket afterwards; jcap doubles /
+    // (NC*ND) entries at most (else device atomics)
+    double *jbuf = nullptr;
+    long long *joff = nullptr;
+    int *jhi = nullptr;  // per ket: end of the bra range holding it [ket.n]
+    long long jcap = 0;
     int krows = 0;  // direct mode: bra exchange rows summed in LDS
 };
 
