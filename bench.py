@@ -42,6 +42,28 @@ def load_xyz(path):
     return sp, np.array(xyz) * ANG2BOHR
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "jk_direct_h2o64_sto3g_hbm_traffic.md")
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` (e.g. "jk_direct_kernel (ps|ss) fp32")
+    from the committed rocprofv3 --pmc summary: FETCH_SIZE and WRITE_SIZE per
+    dispatch, averaged over the kernel's dispatches in the mixed SCF, each
+    scaled by what the counter reads for a plain copy of known size.  Byte
+    counters cannot be read inside a timed run, so this is the profiled
+    figure of the same command, or None if the summary lacks the kernel."""
+    try:
+        with open(TRAFFIC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic_json:"):
+                    table = json.loads(line.split(":", 1)[1])
+                    if kernel in table:
+                        return float(table[kernel]["bytes_per_launch"]), os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def cpu_baseline(species, xyz, n_builds, target_s):
     """Oracle direct Fock build (C, OpenMP over all host threads) on a strided
     sample of the screened shell quartets, scaled to one full build."""
@@ -133,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batch", type=int, default=4096,
                     help="instances of the batched water-cluster workload (0: skip)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (the byte-counter "
+                         "calibration kernel for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -145,6 +170,8 @@ def main():
     if pyfacc.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_mib > 0:
+        pyfacc._native.device_copy(args.copy_mib << 20)
     species, xyz = load_xyz(args.xyz)
     nelec = sum(pyfacc.helper_functions.periodic_table[s] for s in species)
     nocc = nelec // 2
@@ -184,6 +211,8 @@ def main():
     fl = flops.class_flops(cls, q_per_launch, ndens=1, taylor_terms=7 if prec == 0 else 5, orient=orient[cls],
                            prim_quartets=p_mx[prec, cls] / max(launches, 1))
     achieved = fl / (avg_ms * 1e-3) / 1e12
+    kname = f"jk_direct_kernel {flops.CLASS_NAMES[cls]} {'fp64' if prec == 0 else 'fp32'}"
+    traffic, traffic_src = committed_traffic(kname)
     # the same kernel class in the plain FP64 SCF (full builds)
     k64 = int(np.argmax(ms_64[0]))
     avg64 = ms_64[0, k64] / max(n_64[0, k64], 1)
@@ -234,12 +263,13 @@ def main():
         "fp64_build_per_class": per_class,
         "roofline": {
             "bound": "mfma",
-            "kernel": f"jk_direct_kernel {flops.CLASS_NAMES[cls]} {'fp64' if prec == 0 else 'fp32'}",
+            "kernel": kname,
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS if prec == 0 else 157.3,
             "unit": "TFLOP/s",
             "frac": achieved / (PEAK_FP64_TFLOPS if prec == 0 else 157.3),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "avg_launch_ms": avg_ms,
             "launches": launches,
             "flops_per_launch": fl,
