@@ -127,6 +127,7 @@ class Engine {
     std::array<DevBuf<int>, 6> store_nket_;
     // work
     DevBuf<double> w1_, w2_, w3_, state_;
+    int sp2_hint_ = 7;  // purification steps the last projector needed
     // converged densities of the last SCF (for the gradient)
     DevBuf<double> Dfin_;
     bool have_fin_ = false, uhf_fin_ = false;

@@ -379,7 +379,7 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
         gemm_f64_mnk(n, n, nocc, false, true, occ, w1_.p, n, w1_.p, n, 0.0, D, n, st_);
         return;
     }
-    sp2_density(n, w2_.p, double(nocc), w3_.p, w1_.p, state_.p, st_);
+    sp2_hint_ = sp2_density(n, w2_.p, double(nocc), w3_.p, w1_.p, state_.p, st_, 100, sp2_hint_ + 1);
     gemm_f64(n, false, false, 1.0, X_.p, w3_.p, 0.0, w1_.p, st_);       // X P
     gemm_f64(n, false, true, occ, w1_.p, X_.p, 0.0, D, st_);            // occ X P X^T
 }

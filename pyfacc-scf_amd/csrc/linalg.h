@@ -35,16 +35,20 @@ void dev_shell_dmax(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const 
 // columns are eigenvectors.  Synchronises the stream.
 void jacobi_eigh(int n, const double *A, double *w, double *V, hipStream_t st, int max_sweeps = 30,
                  double tol = 1e-15);
-// Spectral projector onto the nocc lowest eigenvectors of symmetric Fp;
-// X receives the projector, work is n x n scratch, state holds
-// sp2_state_size(n) doubles.  Returns the number of iterations used.
 // S^{-1/2} of a symmetric positive definite S (coupled Newton-Schulz on FP64
 // MFMA GEMMs) and an estimate of lambda_min(S) (relative error < 1e-6).
 // Returns false, leaving X undefined, when the iteration has not converged
 // (S singular or nearly so); synchronises the stream.
 bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_t st);
+// Spectral projector onto the nocc lowest eigenvectors of symmetric Fp;
+// X receives the projector, work is n x n scratch, state holds
+// sp2_state_size(n) doubles.  Steps whose predecessor converged do no work,
+// so the host launches first_check steps before it first reads the
+// convergence flag (one stream synchronisation), then checks every 4 steps.
+// Returns the number of purification steps the projector needed (a good
+// first_check for the next, similar matrix is that plus one).
 size_t sp2_state_size(int n);
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
-                int max_iter = 100);
+                int max_iter = 100, int first_check = 8);
 
 }  // namespace pf

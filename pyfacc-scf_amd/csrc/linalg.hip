@@ -29,24 +29,24 @@ namespace pf {
 // Row-major: element (r, c) of A is A[r*lda + c]; op(A) is m x k.
 // C = alpha op(A) op(B) + beta Cin (Cin may alias C).  alpha_p / beta_p, when
 // non-null, are read from device memory (device-side branch selection).
-// One 256-thread workgroup per 32x32 tile of C: the four wavefronts split K
-// (interleaved steps of 4), each accumulating the 2x2 grid of 16x16 MFMA
-// blocks from operands loaded straight from L2, then the four partial tiles
-// are summed through LDS.  For n = 448 this is 196 tiles x 4 wavefronts.
-__global__ void __launch_bounds__(256) gemm_f64_kernel(int m, int n, int k, int ta, int tb, double alpha,
-                                                        const double *__restrict__ alpha_p,
-                                                        const double *__restrict__ A, int lda,
-                                                        const double *__restrict__ B, int ldb, double beta,
-                                                        const double *__restrict__ beta_p, const double *Cin,
-                                                        double *C, int ldc) {
-    __shared__ double red[4][16][64];
+// One workgroup of GW wavefronts per 32x32 tile of C; the wavefronts split K
+// (runs of 4 MFMA k-steps, operand loads of a run issued together) and their
+// partial tiles are summed through LDS.  For n = 448 this is 196 tiles x 8
+// wavefronts.
+constexpr int GW = 8;
+__global__ void __launch_bounds__(64 * GW) gemm_f64_kernel(int m, int n, int k, int ta, int tb, double alpha,
+                                                            const double *__restrict__ alpha_p,
+                                                            const double *__restrict__ A, int lda,
+                                                            const double *__restrict__ B, int ldb, double beta,
+                                                            const double *__restrict__ beta_p, const double *Cin,
+                                                            double *C, int ldc) {
+    __shared__ double red[GW][16][64];
     const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
-    mfma_tile(m, n, k, ta, tb, A, lda, B, ldb, r0, c0, red);
+    mfma_tile_w<GW, 4>(m, n, k, ta, tb, A, lda, B, ldb, r0, c0, red);
     const double al = alpha_p ? *alpha_p : alpha, be = beta_p ? *beta_p : beta;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
-        const double v = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+    for (int idx = threadIdx.x; idx < 1024; idx += 64 * GW) {
+        const int reg = idx >> 6, ln = idx & 63;
+        const double v = red_sum<GW>(red, reg, ln);
         int gr, gc;
         tile_pos(idx, r0, c0, gr, gc);
         if (gr < m && gc < n) {
@@ -61,7 +61,7 @@ void gemm_f64_ex(int m, int n, int k, bool ta, bool tb, double alpha, const doub
                  hipStream_t st) {
     if (m <= 0 || n <= 0) return;
     dim3 grid((n + 31) / 32, (m + 31) / 32);
-    gemm_f64_kernel<<<grid, 256, 0, st>>>(m, n, k, ta, tb, alpha, alpha_p, A, lda, B, ldb, beta, beta_p, Cin, C, ldc);
+    gemm_f64_kernel<<<grid, 64 * GW, 0, st>>>(m, n, k, ta, tb, alpha, alpha_p, A, lda, B, ldb, beta, beta_p, Cin, C, ldc);
     PF_HIP(hipGetLastError());
 }
 
@@ -607,10 +607,10 @@ __global__ void __launch_bounds__(256) sp2_init_kernel(int n, const double *F, d
 // |t1 - t2| is below tol (or stops shrinking below 1e-8, after step 2) the
 // step copies X and the done flag is raised.  X' partials and records go to
 // slot (k+1) % 3.
-__global__ void __launch_bounds__(256) sp2_step_kernel(int n, double nocc, double tol, int k, const double *X,
+__global__ void __launch_bounds__(64 * GW) sp2_step_kernel(int n, double nocc, double tol, int k, const double *X,
                                                         double *Xo, double *state) {
-    __shared__ double red[4][16][64];
-    __shared__ double sh[4];
+    __shared__ double red[GW][16][64];
+    __shared__ double sh[GW];
     const int nt = (n + 31) / 32, tile = blockIdx.y * nt + blockIdx.x, ntile = nt * nt;
     const int s0 = k % 3, s1 = (k + 1) % 3;
     // record [3 + 2s]: steps since convergence (0 = not converged).  The first
@@ -625,22 +625,21 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(int n, double nocc, doubl
     }
     const double *P = state + 8;
     double t1 = 0.0, t2 = 0.0;
-    for (int b = threadIdx.x; b < ntile; b += 256) {  // fixed order: deterministic sums
+    for (int b = threadIdx.x; b < ntile; b += 64 * GW) {  // fixed order: deterministic sums
         t1 += P[(2 * s0 + 0) * ntile + b];
         t2 += P[(2 * s0 + 1) * ntile + b];
     }
-    t1 = block_sum256(t1, sh);
-    t2 = block_sum256(t2, sh);
+    t1 = block_sum<GW>(t1, sh);
+    t2 = block_sum<GW>(t2, sh);
     const double err = fabs(t1 - t2), prev = state[2 + 2 * s0];
     bool done = age != 0.0;
     if (!done && k > 2 && (err < tol || (err < 1e-8 && err >= prev))) done = true;
     const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
     const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    if (!done) mfma_tile(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
+    if (!done) mfma_tile_w<GW, 4>(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
     double u1 = 0.0, u2 = 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = threadIdx.x + 256 * q, reg = idx >> 6, ln = idx & 63;
+    for (int idx = threadIdx.x; idx < 1024; idx += 64 * GW) {
+        const int reg = idx >> 6, ln = idx & 63;
         int gr, gc;
         tile_pos(idx, r0, c0, gr, gc);
         if (gr < n && gc < n) {
@@ -649,7 +648,7 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(int n, double nocc, doubl
             if (done) {
                 v = X[o];
             } else {
-                const double x2 = red[0][reg][ln] + red[1][reg][ln] + red[2][reg][ln] + red[3][reg][ln];
+                const double x2 = red_sum<GW>(red, reg, ln);
                 v = square ? x2 : 2.0 * X[o] - x2;
             }
             Xo[o] = v;
@@ -657,8 +656,8 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(int n, double nocc, doubl
             if (gr == gc) u1 += v;
         }
     }
-    u1 = block_sum256(u1, sh);
-    u2 = block_sum256(u2, sh);
+    u1 = block_sum<GW>(u1, sh);
+    u2 = block_sum<GW>(u2, sh);
     if (threadIdx.x == 0) {
         state[8 + (2 * s1 + 0) * ntile + tile] = u1;
         state[8 + (2 * s1 + 1) * ntile + tile] = u2;
@@ -670,7 +669,7 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(int n, double nocc, doubl
 }
 
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
-                int max_iter) {
+                int max_iter, int first_check) {
     const int nt = (n + 31) / 32;
     const dim3 tiles(nt, nt);
     double *rows = state + 8 + 6 * (size_t)sp2_tiles(n);
@@ -679,15 +678,20 @@ int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, d
     sp2_init_kernel<<<tiles, 256, 0, st>>>(n, Fp, state, X);
     const double tol = 1e-13 * std::max(1.0, nocc);
     double *cur = X, *nxt = work;
-    int it = 0;
-    for (; it < max_iter; ++it) {
-        sp2_step_kernel<<<tiles, 256, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
+    int used = max_iter, check = std::max(1, std::min(first_check, max_iter));
+    for (int it = 0; it < max_iter; ++it) {
+        sp2_step_kernel<<<tiles, 64 * GW, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
         std::swap(cur, nxt);
-        if ((it & 7) == 7) {
-            double done = 0.0;
-            PF_HIP(hipMemcpyAsync(&done, state + 3 + 2 * ((it + 1) % 3), sizeof(double), hipMemcpyDeviceToHost, st));
+        if (it + 1 == check) {
+            // record written by step it: steps since convergence (0: not yet)
+            double age = 0.0;
+            PF_HIP(hipMemcpyAsync(&age, state + 3 + 2 * ((it + 1) % 3), sizeof(double), hipMemcpyDeviceToHost, st));
             PF_HIP(hipStreamSynchronize(st));
-            if (done != 0.0) break;
+            if (age != 0.0) {
+                used = it + 1 - (int)age;
+                break;
+            }
+            check += 4;
         }
     }
     // symmetrise (X + X^T)/2 into X
@@ -699,7 +703,7 @@ int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, d
         symmetrize_kernel<<<g2, 128, 0, st>>>(n, work, X);
     }
     PF_HIP(hipGetLastError());
-    return it;
+    return used;
 }
 
 }  // namespace pf
