@@ -140,6 +140,9 @@ def batched(pyfacc, count):
         "converged_mixed": int(rmx["converged"].sum()),
         # over the molecules converged in both forms (the others stop at max_iter)
         "max_abs_dE_mixed_vs_fp64": float(np.abs(rmx["energy"] - r64["energy"])[both].max()),
+        # instances whose two precisions land more than 1e-6 Eh apart (a different
+        # UHF stationary point, not rounding: the single-molecule engine agrees)
+        "n_abs_dE_above_1e-6": int((np.abs(rmx["energy"] - r64["energy"])[both] > 1e-6).sum()),
         "hipgraph": bool(st64["graph"]),
         "digest_waves_autotuned": int(st64["digest_waves"]),
     }

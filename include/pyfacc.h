@@ -63,10 +63,13 @@ typedef struct pf_options {
     int diag;             /* PF_DIAG_SP2 (GEMM spectral projector) or PF_DIAG_JACOBI */
     int renormalize;      /* 1: contracted functions normalised (Libint); 0: primitive norms only (reference) */
     double schwarz;       /* screening threshold on Q_ab Q_cd max|D| */
-    double mixed_switch;  /* mixed precision: ||dD||_F below which the FP64 finish starts */
+    double mixed_switch;  /* mixed precision: ||dD||_F below which the FP64 finish starts; one full
+                             FP64 rebuild inside the FP32 phase at 1e4 * mixed_switch (pf_scf) */
     int verbose;          /* 1: one line per iteration on stderr */
     double schwarz_inc;   /* screening threshold of FP64 incremental builds (on Q Q max|dD|) */
     int rebuild_every;    /* incremental FP64 builds: full rebuild after this many increments (0: never) */
+    double schwarz_fp32;  /* mixed precision: screening threshold of the FP32 incremental builds (on
+                             Q Q max|dD|; 0: schwarz).  Their error is wiped by the full FP64 builds. */
 } pf_options;
 
 typedef struct pf_result {
@@ -91,7 +94,7 @@ typedef struct pf_result {
 
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
- * rebuild_every 0. */
+ * rebuild_every 0, schwarz_fp32 0 (= schwarz). */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

@@ -443,6 +443,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     for (int k = 0; k < ndiis; ++k) { dF[k].alloc(n2 * nd); dE[k].alloc(n2 * nd); }
     std::vector<double> Bmat((size_t)ndiis * ndiis, 0.0);
     DevBuf<double> dots(std::max(ndiis, 1)), Fx(ndiis ? n2 * nd : 1);
+    DevBuf<double> dots_all(opt.precision == PF_PREC_MIXED && ndiis ? (size_t)ndiis * ndiis : 1);
     int nstore = 0, head = 0;
 
     bool fp32_phase = opt.precision == PF_PREC_MIXED;
@@ -450,31 +451,82 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     const int gmode = opt.eri_mode == PF_ERI_STORED ? MODE_LOAD : MODE_DIRECT;
     double Eold = 0.0, Eel = 0.0, last_dD = 1e300;
     int it = 0, since_full = 0;
+    bool anchored = false;
     bool conv = false;
     for (it = 1; it <= opt.max_iter; ++it) {
         // ---- Fock build
         const double tf = now();
         const double *Dcur0 = D.p, *Dcur1 = uhf ? D.p + n2 : nullptr;
         double *G0 = G.p, *G1 = uhf ? G.p + n2 : nullptr;
+        // Mixed precision: the FP32 rounding of an incremental build scales with
+        // |dD|, so the first builds (dD as large as D itself) leave almost all
+        // of the accumulated error.  One full FP64 rebuild when ||dD|| first
+        // falls below MIXED_ANCHOR * mixed_switch clears it; the switch to FP64
+        // at mixed_switch then moves the density by little more than a normal
+        // step.
         const bool switch_now = fp32_phase && last_dD < opt.mixed_switch;
+        const bool anchor_now = fp32_phase && !switch_now && !anchored && last_dD < MIXED_ANCHOR * opt.mixed_switch;
         const bool rebuild = !fp32_phase && incremental && opt.rebuild_every > 0 &&
                              since_full >= opt.rebuild_every;
-        if (switch_now || rebuild || !incremental) {
-            // full FP64 build; in mixed precision this is the switch to FP64, after
-            // which the DIIS subspace (FP32-era Fock matrices) restarts
-            if (switch_now) {
-                fp32_phase = false;
-                nstore = 0;
-                head = 0;
+        if (switch_now || anchor_now || rebuild || !incremental) {
+            // full FP64 build: the anchor of the FP32 phase, or the switch to FP64
+            DevBuf<double> Gs;
+            // DIIS keeps its subspace: at the anchor the error vectors (~|dD|) dwarf
+            // the offset, and those entries leave the subspace before it matters;
+            // at a switch with no anchor before it, the offset is removed below
+            const bool correct = switch_now && !anchored;
+            if (anchor_now) anchored = true;
+            if (switch_now) fp32_phase = false;
+            if (correct) {
+                if (nstore > 0) {
+                    // the FP32 phase's G at the current density (one more incremental
+                    // FP32 build), to measure the offset its accumulated rounding left
+                    Gs.alloc(n2 * nd);
+                    PF_HIP(hipMemcpyAsync(Gs.p, G.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+                    dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
+                    res.quartets += build_g<float>(dD.p, uhf ? dD.p + n2 : nullptr, Gs.p, uhf ? Gs.p + n2 : nullptr,
+                                                   true, MODE_DIRECT);
+                    ++res.fock_builds_fp32;
+                }
             }
             res.quartets += build_g<double>(Dcur0, Dcur1, G0, G1, false, gmode);
             ++res.fock_builds_fp64;
             since_full = 0;
+            if (Gs.p) {
+                // The stored DIIS Fock matrices share (nearly) this offset
+                // d = G_fp32 - G_fp64, accumulated mostly by the first builds:
+                // remove it from each F_k
+                // and its error vector, e_k -= d D S - S D d (with the current D for
+                // D_k, an error of |d| |D_k - D|), and rebuild the B matrix.  The
+                // subspace then continues instead of restarting.
+                dev_axpby(n2 * nd, 1.0, Gs.p, -1.0, G.p, Gs.p, st_);
+                for (int s = 0; s < nd; ++s) {
+                    gemm_f64(n, false, false, 1.0, Gs.p + s * n2, D.p + s * n2, 0.0, w1_.p, st_);
+                    gemm_f64(n, false, false, 1.0, w1_.p, S_.p, 0.0, w2_.p, st_);
+                    dev_antisym(n, w2_.p, dD.p + s * n2, st_);
+                }
+                for (int k = 0; k < nstore; ++k) {
+                    dev_axpby(n2 * nd, 1.0, dF[k].p, -1.0, Gs.p, dF[k].p, st_);
+                    dev_axpby(n2 * nd, 1.0, dE[k].p, -1.0, dD.p, dE[k].p, st_);
+                }
+                for (int r = 0; r < nstore; ++r)
+                    for (int k0 = 0; k0 < nstore; k0 += DotPtrs::MAX) {
+                        const int m = std::min(DotPtrs::MAX, nstore - k0);
+                        DotPtrs vecs{};
+                        for (int k = 0; k < m; ++k) vecs.p[k] = dE[k0 + k].p;
+                        dev_dots(n2 * nd, dE[r].p, vecs, m, dots_all.p + (size_t)r * ndiis + k0, part.p, st_);
+                    }
+                std::vector<double> b((size_t)nstore * ndiis);
+                dots_all.download(b.data(), b.size(), st_);
+                PF_HIP(hipStreamSynchronize(st_));
+                for (int r = 0; r < nstore; ++r)
+                    for (int k = 0; k < nstore; ++k) Bmat[r * ndiis + k] = b[(size_t)r * ndiis + k];
+            }
         } else {
             dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
             const double *d0 = dD.p, *d1 = uhf ? dD.p + n2 : nullptr;
             if (fp32_phase) {
-                res.quartets += build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT);
+                res.quartets += build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, opt.schwarz_fp32);
                 ++res.fock_builds_fp32;
             } else {
                 res.quartets += build_g<double>(d0, d1, G0, G1, true, gmode, opt.schwarz_inc);
