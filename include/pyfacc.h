@@ -68,8 +68,8 @@ typedef struct pf_options {
     int verbose;          /* 1: one line per iteration on stderr */
     double schwarz_inc;   /* screening threshold of FP64 incremental builds (on Q Q max|dD|) */
     int rebuild_every;    /* incremental FP64 builds: full rebuild after this many increments (0: never) */
-    double schwarz_fp32;  /* mixed precision: screening threshold of the FP32 incremental builds (on
-                             Q Q max|dD|; 0: schwarz).  Their error is wiped by the full FP64 builds. */
+    double schwarz_fp32;  /* mixed precision: screening threshold (on Q Q max|dD|; 0: schwarz) of the
+                             FP32 incremental builds before the full FP64 rebuild that replaces their sum */
 } pf_options;
 
 typedef struct pf_result {
@@ -94,7 +94,7 @@ typedef struct pf_result {
 
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
- * rebuild_every 0, schwarz_fp32 0 (= schwarz). */
+ * rebuild_every 0, schwarz_fp32 1e-6. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

@@ -94,7 +94,7 @@ void pf_default_options(pf_options *o) {
     o->verbose = 0;
     o->schwarz_inc = 1e-12;
     o->rebuild_every = 0;
-    o->schwarz_fp32 = 0.0;
+    o->schwarz_fp32 = 1e-6;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }

@@ -526,7 +526,8 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
             dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
             const double *d0 = dD.p, *d1 = uhf ? dD.p + n2 : nullptr;
             if (fp32_phase) {
-                res.quartets += build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, opt.schwarz_fp32);
+                // before the anchor rebuild (which replaces G) screening may be looser
+                res.quartets += build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, anchored ? 0.0 : opt.schwarz_fp32);
                 ++res.fock_builds_fp32;
             } else {
                 res.quartets += build_g<double>(d0, d1, G0, G1, true, gmode, opt.schwarz_inc);
