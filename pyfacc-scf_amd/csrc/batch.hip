@@ -449,7 +449,8 @@ void BatchEngine::setup() {
     };
     M_ = (int)mols_.size();
     if (M_ == 0) throw Error("batch: no molecules");
-    const BasisBlocks blocks = parse_basis_blocks(read_basis_file(basis_path_), basis_path_);
+    const auto blocks_p = basis_blocks_cached(basis_path_);
+    const BasisBlocks &blocks = *blocks_p;
     std::vector<int> shoff(M_ + 1, 0), aoff(M_ + 1, 0);
     std::vector<double> axyz;
     std::vector<int> Z;

@@ -88,9 +88,14 @@ class Engine {
     void setup_grad_pairs();
     // G (FP64) += or = G[D] in precision R.  D0/D1 device FP64 densities (D1 null for RHF).
     // tau > 0 overrides the Schwarz threshold (used for incremental builds).
+    // Nothing returns to the host: quartet counts accumulate on the device and
+    // the class timings in recorded events, both read by collect_profile.
     template <typename R>
-    unsigned long long build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
-                               int mode, double tau = 0.0);
+    void build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate, int mode,
+                 double tau = 0.0);
+    // Synchronises the stream, adds the builds since the last call to the
+    // prof_* statistics and returns their shell-quartet count.
+    unsigned long long collect_profile();
     void density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
                            double *eps_dev);
 
@@ -113,11 +118,16 @@ class Engine {
     DevBuf<double> J64_, K64a_, K64b_, Dsh64_, dmax64_;
     DevBuf<float> Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;
     DevBuf<double> Dj64_;
-    DevBuf<unsigned long long> count_;
+    DevBuf<unsigned long long> count_;  // [precision][quartets 6, primitive quartets 6]
+    struct PendingBuild {
+        int pbase;
+        hipEvent_t ev[7];
+    };
+    std::vector<PendingBuild> pending_;  // builds whose class timings are not yet read
     std::array<DevBuf<int>, 6> plan_nket_, plan_off_, plan_work_;  // direct-mode work plans per class
     std::array<DevBuf<long long>, 6> plan_joff_;
     std::array<DevBuf<int>, 6> plan_jhi_;
-    std::array<DevBuf<double>, 6> jbuf_;  // per class: J'_cd of each evaluated quartet
+    std::array<PooledBlock, 6> jbuf_;  // per class: J'_cd of each evaluated quartet (pooled)
     std::array<long long, 6> jcap_{};
     static constexpr long long JBUF_MAX_VALUES = 1LL << 27;  // per class (1 GiB), beyond: device atomics
     // mixed precision: full FP64 rebuild once ||dD|| < MIXED_ANCHOR * mixed_switch

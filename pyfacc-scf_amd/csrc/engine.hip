@@ -63,9 +63,9 @@ Engine::Engine(const Molecule &mol, const std::vector<std::string> &species, con
     d_shells_.upload(basis_.shells.data(), nsh, st_);
     d_axyz_.upload(mol_.xyz.data(), mol_.xyz.size(), st_);
     d_Z_.upload(mol_.Z.data(), mol_.Z.size(), st_);
-    const auto tab = boys_table_host();
+    static const std::vector<double> tab = boys_table_host();  // computed once per process
+    static const std::vector<float> ftab(tab.begin(), tab.end());
     d_tab_.upload(tab.data(), tab.size(), st_);
-    std::vector<float> ftab(tab.begin(), tab.end());
     f_tab_.upload(ftab.data(), ftab.size(), st_);
     std::vector<int> bf(nsh), nf(nsh);
     for (int s = 0; s < nsh; ++s) { bf[s] = basis_.shells[s].bf; nf[s] = ncart(basis_.shells[s].l); }
@@ -79,6 +79,7 @@ Engine::~Engine() {
         (void)hipStreamSynchronize(st_);
         (void)hipStreamDestroy(st_);
     }
+    for (auto &b : jbuf_) pool_release(b);  // the stream is idle: nothing pending uses them
 }
 
 void Engine::setup_one_electron() {
@@ -164,7 +165,8 @@ void Engine::setup_pairs() {
     const size_t n2 = (size_t)nbf() * nbf();
     J64_.alloc(n2); K64a_.alloc(n2); K64b_.alloc(n2); Dj64_.alloc(n2);
     Dsh64_.alloc((size_t)nshell() * nshell()); dmax64_.alloc(1);
-    count_.alloc(12);
+    count_.alloc(24);
+    PF_HIP(hipMemsetAsync(count_.p, 0, 24 * sizeof(unsigned long long), st_));
     for (size_t q = 0; q < classes_.size(); ++q) {  // direct-mode work plans
         const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
         const int nb = std::max(B.n, 1);
@@ -174,7 +176,7 @@ void Engine::setup_pairs() {
         const long long full = same ? (long long)B.n * (B.n + 1) / 2 : (long long)B.n * K.n;
         const long long ncd = (long long)ncart(K.la) * ncart(K.lb);
         jcap_[q] = std::min<long long>(full, JBUF_MAX_VALUES / ncd);
-        jbuf_[q].alloc(std::max<long long>(jcap_[q] * ncd, 1));
+        jbuf_[q] = pool_acquire(sizeof(double) * (size_t)std::max<long long>(jcap_[q] * ncd, 1));
         plan_jhi_[q].alloc(std::max(K.n, 1));
     }
     PF_HIP(hipStreamSynchronize(st_));
@@ -224,7 +226,7 @@ void Engine::setup_store() {
 }
 
 template <typename R>
-unsigned long long Engine::build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
+void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1, bool accumulate,
                                    int mode, double tau) {
     setup_pairs();
     const int n = nbf(), nsh = nshell();
@@ -264,11 +266,11 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
         a.Dsh = Dsh32_.p; a.dmax = dmax32_.p;
     }
     a.J = J; a.K0 = Ka; a.K1 = uhf ? Kb : nullptr;
-    PF_HIP(hipMemsetAsync(count_.p, 0, 12 * sizeof(unsigned long long), st_));
-    hipEvent_t ev[7];
+    const int pbase = std::is_same<R, double>::value ? 0 : 6;
+    PendingBuild pb{pbase, {}};
+    hipEvent_t *ev = pb.ev;
     if (profile_classes)
-        for (auto &e : ev) PF_HIP(hipEventCreate(&e));
-    static_assert(sizeof(ev) / sizeof(ev[0]) == 7, "one event per class + 1");
+        for (int e = 0; e < 7; ++e) PF_HIP(hipEventCreate(&ev[e]));
     for (size_t q = 0; q < classes_.size(); ++q) {
         const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
         if constexpr (std::is_same<R, double>::value) { a.bra = B.view64(); a.ket = K.view64(); }
@@ -279,35 +281,42 @@ unsigned long long Engine::build_g(const double *D0, const double *D1, double *G
                 a.store = store_[q].p; a.store_off = store_off_[q].p; a.store_nket = store_nket_[q].p;
             }
         }
-        a.count = count_.p + classes_[q].slot;
-        a.pcount = count_.p + 6 + classes_[q].slot;
+        a.count = count_.p + 2 * pbase + classes_[q].slot;
+        a.pcount = count_.p + 2 * pbase + 6 + classes_[q].slot;
         a.nket = plan_nket_[q].p; a.item_off = plan_off_[q].p; a.work = plan_work_[q].p;
-        a.joff = plan_joff_[q].p; a.jbuf = jbuf_[q].p; a.jcap = jcap_[q]; a.jhi = plan_jhi_[q].p;
+        a.joff = plan_joff_[q].p; a.jbuf = static_cast<double *>(jbuf_[q].p); a.jcap = jcap_[q]; a.jhi = plan_jhi_[q].p;
         if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
     if (profile_classes) PF_HIP(hipEventRecord(ev[6], st_));
     dev_assemble_g<double>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
-    unsigned long long cnt6[12] = {0};
-    PF_HIP(hipMemcpyAsync(cnt6, count_.p, sizeof(cnt6), hipMemcpyDeviceToHost, st_));
+    for (int q = 0; q < 6; ++q) prof_launches[pbase + q] += 1;
+    if (profile_classes) pending_.push_back(pb);
+}
+
+unsigned long long Engine::collect_profile() {
+    if (!count_.p) return 0;
+    unsigned long long c[24];
+    PF_HIP(hipMemcpyAsync(c, count_.p, sizeof(c), hipMemcpyDeviceToHost, st_));
     PF_HIP(hipStreamSynchronize(st_));
-    const int pbase = std::is_same<R, double>::value ? 0 : 6;
-    unsigned long long cnt = 0;
-    for (int q = 0; q < 6; ++q) {
-        prof_quartets[pbase + q] += cnt6[q];
-        prof_prims[pbase + q] += cnt6[6 + q];
-        prof_launches[pbase + q] += 1;
-        cnt += cnt6[q];
-    }
-    if (profile_classes) {
+    PF_HIP(hipMemsetAsync(count_.p, 0, sizeof(c), st_));
+    unsigned long long total = 0;
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 6; ++q) {
+            prof_quartets[6 * p + q] += c[12 * p + q];
+            prof_prims[6 * p + q] += c[12 * p + 6 + q];
+            total += c[12 * p + q];
+        }
+    for (auto &b : pending_) {
         for (int q = 0; q < 6; ++q) {
             float ms = 0.f;
-            PF_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
-            prof_ms[pbase + classes_[q].slot] += ms;
+            PF_HIP(hipEventElapsedTime(&ms, b.ev[q], b.ev[q + 1]));
+            prof_ms[b.pbase + classes_[q].slot] += ms;
         }
-        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : b.ev) (void)hipEventDestroy(e);
     }
-    return cnt;
+    pending_.clear();
+    return total;
 }
 
 void Engine::jk_host(int ndens, const double *D, double *J, double *K, int mode, int precision) {
@@ -441,8 +450,9 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     const int ndiis = opt.diis > 1 ? std::min(opt.diis, 32) : 0;
     std::vector<DevBuf<double>> dF(ndiis), dE(ndiis);
     for (int k = 0; k < ndiis; ++k) { dF[k].alloc(n2 * nd); dE[k].alloc(n2 * nd); }
-    std::vector<double> Bmat((size_t)ndiis * ndiis, 0.0);
-    DevBuf<double> dots(std::max(ndiis, 1)), Fx(ndiis ? n2 * nd : 1);
+    DevBuf<double> dots(std::max(ndiis, 1)), Fx(ndiis ? n2 * nd : 1), Bdev(std::max(ndiis * ndiis, 1)),
+        coef(std::max(ndiis, 1));
+    DevBuf<int> dok(1);
     DevBuf<double> dots_all(opt.precision == PF_PREC_MIXED && ndiis ? (size_t)ndiis * ndiis : 1);
     int nstore = 0, head = 0;
 
@@ -453,9 +463,19 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     int it = 0, since_full = 0;
     bool anchored = false;
     bool conv = false;
+    // per iteration: events at the start of the Fock build, its end, and the end
+    // of the density/energy step (t_fock and t_diag without host round trips)
+    std::vector<std::array<hipEvent_t, 3>> tev;
+    auto mark = [&](int k) {
+        if (k == 0) {
+            tev.emplace_back();
+            for (auto &e : tev.back()) PF_HIP(hipEventCreate(&e));
+        }
+        PF_HIP(hipEventRecord(tev.back()[k], st_));
+    };
     for (it = 1; it <= opt.max_iter; ++it) {
         // ---- Fock build
-        const double tf = now();
+        mark(0);
         const double *Dcur0 = D.p, *Dcur1 = uhf ? D.p + n2 : nullptr;
         double *G0 = G.p, *G1 = uhf ? G.p + n2 : nullptr;
         // Mixed precision: the FP32 rounding of an incremental build scales with
@@ -484,12 +504,12 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                     Gs.alloc(n2 * nd);
                     PF_HIP(hipMemcpyAsync(Gs.p, G.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
                     dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
-                    res.quartets += build_g<float>(dD.p, uhf ? dD.p + n2 : nullptr, Gs.p, uhf ? Gs.p + n2 : nullptr,
-                                                   true, MODE_DIRECT);
+                    build_g<float>(dD.p, uhf ? dD.p + n2 : nullptr, Gs.p, uhf ? Gs.p + n2 : nullptr, true,
+                                   MODE_DIRECT);
                     ++res.fock_builds_fp32;
                 }
             }
-            res.quartets += build_g<double>(Dcur0, Dcur1, G0, G1, false, gmode);
+            build_g<double>(Dcur0, Dcur1, G0, G1, false, gmode);
             ++res.fock_builds_fp64;
             since_full = 0;
             if (Gs.p) {
@@ -516,31 +536,26 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                         for (int k = 0; k < m; ++k) vecs.p[k] = dE[k0 + k].p;
                         dev_dots(n2 * nd, dE[r].p, vecs, m, dots_all.p + (size_t)r * ndiis + k0, part.p, st_);
                     }
-                std::vector<double> b((size_t)nstore * ndiis);
-                dots_all.download(b.data(), b.size(), st_);
-                PF_HIP(hipStreamSynchronize(st_));
-                for (int r = 0; r < nstore; ++r)
-                    for (int k = 0; k < nstore; ++k) Bmat[r * ndiis + k] = b[(size_t)r * ndiis + k];
+                PF_HIP(hipMemcpyAsync(Bdev.p, dots_all.p, sizeof(double) * (size_t)nstore * ndiis,
+                                      hipMemcpyDeviceToDevice, st_));
             }
         } else {
             dev_axpby(n2 * nd, 1.0, D.p, -1.0, Dbuilt.p, dD.p, st_);
             const double *d0 = dD.p, *d1 = uhf ? dD.p + n2 : nullptr;
             if (fp32_phase) {
                 // before the anchor rebuild (which replaces G) screening may be looser
-                res.quartets += build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, anchored ? 0.0 : opt.schwarz_fp32);
+                build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, anchored ? 0.0 : opt.schwarz_fp32);
                 ++res.fock_builds_fp32;
             } else {
-                res.quartets += build_g<double>(d0, d1, G0, G1, true, gmode, opt.schwarz_inc);
+                build_g<double>(d0, d1, G0, G1, true, gmode, opt.schwarz_inc);
                 ++res.fock_builds_fp64;
                 ++since_full;
             }
         }
         if (incremental) PF_HIP(hipMemcpyAsync(Dbuilt.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
         for (int s = 0; s < nd; ++s) dev_axpby(n2, 1.0, H_.p, 1.0, G.p + s * n2, F.p + s * n2, st_);
-        PF_HIP(hipStreamSynchronize(st_));
-        res.t_fock += now() - tf;
+        mark(1);
 
-        const double td = now();
         PF_HIP(hipMemcpyAsync(Dold.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
         const double *Fuse = F.p;
         if (ndiis) {
@@ -560,45 +575,15 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                 for (int k = 0; k < m; ++k) vecs.p[k] = dE[k0 + k].p;
                 dev_dots(n2 * nd, dE[slot].p, vecs, m, dots.p + k0, part.p, st_);
             }
-            std::vector<double> row(nstore);
-            dots.download(row.data(), nstore, st_);
-            PF_HIP(hipStreamSynchronize(st_));
-            for (int k = 0; k < nstore; ++k) { Bmat[slot * ndiis + k] = row[k]; Bmat[k * ndiis + slot] = row[k]; }
             if (nstore >= 2) {
-                // solve [B -1; -1 0] [c; l] = [0; -1] (Gaussian elimination with pivoting)
-                const int m = nstore, N = m + 1;
-                std::vector<int> idx;
-                for (int k = 0; k < ndiis && (int)idx.size() < m; ++k)
-                    if (k < nstore) idx.push_back(k);
-                std::vector<double> M((size_t)N * (N + 1), 0.0);
-                for (int r = 0; r < m; ++r) {
-                    for (int c = 0; c < m; ++c) M[r * (N + 1) + c] = Bmat[idx[r] * ndiis + idx[c]];
-                    M[r * (N + 1) + m] = -1.0;
-                }
-                for (int c = 0; c < m; ++c) M[m * (N + 1) + c] = -1.0;
-                M[m * (N + 1) + N] = -1.0;
-                bool ok = true;
-                for (int col = 0; col < N && ok; ++col) {
-                    int piv = col;
-                    for (int r = col + 1; r < N; ++r)
-                        if (std::fabs(M[r * (N + 1) + col]) > std::fabs(M[piv * (N + 1) + col])) piv = r;
-                    if (std::fabs(M[piv * (N + 1) + col]) < 1e-300) { ok = false; break; }
-                    if (piv != col)
-                        for (int k = 0; k <= N; ++k) std::swap(M[col * (N + 1) + k], M[piv * (N + 1) + k]);
-                    for (int r = 0; r < N; ++r) {
-                        if (r == col) continue;
-                        const double f = M[r * (N + 1) + col] / M[col * (N + 1) + col];
-                        for (int k = col; k <= N; ++k) M[r * (N + 1) + k] -= f * M[col * (N + 1) + k];
-                    }
-                }
-                if (ok) {
-                    dev_fill(n2 * nd, 0.0, Fx.p, st_);
-                    for (int r = 0; r < m; ++r) {
-                        const double c = M[r * (N + 1) + N] / M[r * (N + 1) + r];
-                        dev_axpby(n2 * nd, 1.0, Fx.p, c, dF[idx[r]].p, Fx.p, st_);
-                    }
-                    Fuse = Fx.p;
-                }
+                // B row/column of this slot, the Pulay system and the extrapolation, all on the device
+                dev_diis_solve(ndiis, nstore, slot, dots.p, Bdev.p, coef.p, dok.p, st_);
+                DiisPtrs vs{};
+                for (int k = 0; k < nstore; ++k) vs.p[k] = dF[k].p;
+                dev_diis_extrap(n2 * nd, nstore, coef.p, dok.p, vs, F.p, Fx.p, st_);
+                Fuse = Fx.p;
+            } else {
+                dev_diis_solve(ndiis, nstore, slot, dots.p, Bdev.p, coef.p, dok.p, st_);  // records the B entries
             }
         }
         for (int s = 0; s < nd; ++s) density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr);
@@ -606,10 +591,10 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         if (!uhf) dev_energy_rhf(n, D.p, H_.p, F.p, red.p, part.p, st_);
         else dev_energy_uhf(n, D.p, D.p + n2, H_.p, F.p, F.p + n2, red.p, part.p, st_);
         dev_diff_sq(n2 * nd, D.p, Dold.p, red.p + 1, part.p, st_);
+        mark(2);
         double hv[2];
         PF_HIP(hipMemcpyAsync(hv, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
         PF_HIP(hipStreamSynchronize(st_));
-        res.t_diag += now() - td;
         Eel = uhf ? hv[0] : 0.5 * hv[0];
         const double dE_ = std::fabs(Eel - Eold), errD = std::sqrt(hv[1]);
         last_dD = errD;
@@ -626,6 +611,15 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     PF_HIP(hipMemcpyAsync(Dfin_.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
     have_fin_ = true;
     uhf_fin_ = uhf;
+    res.quartets = (double)collect_profile();  // also synchronises the stream
+    for (auto &e3 : tev) {
+        float a = 0.f, b = 0.f;
+        PF_HIP(hipEventElapsedTime(&a, e3[0], e3[1]));
+        PF_HIP(hipEventElapsedTime(&b, e3[1], e3[2]));
+        res.t_fock += 1e-3 * a;
+        res.t_diag += 1e-3 * b;
+        for (auto &e : e3) (void)hipEventDestroy(e);
+    }
     res.iterations = std::min(it, opt.max_iter);
     res.converged = conv ? 1 : 0;
     res.e_elec = Eel;
@@ -793,9 +787,9 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
     }
 }
 
-template unsigned long long Engine::build_g<double>(const double *, const double *, double *, double *, bool, int,
+template void Engine::build_g<double>(const double *, const double *, double *, double *, bool, int,
                                                     double);
-template unsigned long long Engine::build_g<float>(const double *, const double *, double *, double *, bool, int,
+template void Engine::build_g<float>(const double *, const double *, double *, double *, bool, int,
                                                    double);
 
 }  // namespace pf

@@ -16,6 +16,20 @@ struct DotPtrs {
 // out[y] = a . b.p[y], y < m <= 8
 void dev_dots(int64_t n, const double *a, const DotPtrs &b, int m, double *out, double *part, hipStream_t st);
 void dev_diff_sq(int64_t n, const double *a, const double *b, double *out, double *part, hipStream_t st);
+// Pulay DIIS on the device (no host round trip).  dev_diis_solve sets
+// B[slot][k] = B[k][slot] = dots[k] (k < nstore; B is ndiis x ndiis) and
+// solves [B -1; -1 0] [c; l] = [0; -1] over the nstore stored vectors by
+// Gaussian elimination with partial pivoting in one workgroup: coef[k], and
+// ok[0] = 0 when nstore < 2 or a pivot vanishes.  dev_diis_extrap writes
+// Fx = sum_k coef[k] v.p[k] when ok[0], else Fx = F.
+struct DiisPtrs {
+    static constexpr int MAX = 32;
+    const double *p[MAX];
+};
+void dev_diis_solve(int ndiis, int nstore, int slot, const double *dots, double *B, double *coef, int *ok,
+                    hipStream_t st);
+void dev_diis_extrap(int64_t n, int nstore, const double *coef, const int *ok, const DiisPtrs &v, const double *F,
+                     double *Fx, hipStream_t st);
 void dev_energy_rhf(int n, const double *D, const double *H, const double *F, double *out, double *part,
                     hipStream_t st);
 void dev_energy_uhf(int n, const double *Da, const double *Db, const double *H, const double *Fa, const double *Fb,

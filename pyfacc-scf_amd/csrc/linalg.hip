@@ -139,6 +139,88 @@ void dev_energy_uhf(int n, const double *Da, const double *Db, const double *H, 
     reduce2((int64_t)n * n, 1, EnergyUF{Da, Db, H, Fa, Fb}, out, part, st);
 }
 
+// ---- DIIS
+__global__ void __launch_bounds__(64) diis_solve_kernel(int ndiis, int nstore, int slot, const double *dots,
+                                                        double *B, double *coef, int *ok) {
+    constexpr int NM = DiisPtrs::MAX + 1;
+    __shared__ double M[NM][NM + 1];
+    __shared__ int piv_s;
+    const int t = threadIdx.x, m = nstore, N = m + 1;
+    if (t < m) {
+        B[slot * ndiis + t] = dots[t];
+        B[t * ndiis + slot] = dots[t];
+    }
+    __syncthreads();
+    if (m < 2) {
+        if (t == 0) ok[0] = 0;
+        return;
+    }
+    // augmented system, row r = lane r: [B -1 | 0] and [-1 .. -1 0 | -1]
+    if (t < N) {
+        for (int c = 0; c < N; ++c)
+            M[t][c] = t < m ? (c < m ? B[t * ndiis + c] : -1.0) : (c < m ? -1.0 : 0.0);
+        M[t][N] = t < m ? 0.0 : -1.0;
+    }
+    __syncthreads();
+    bool good = true;
+    for (int col = 0; col < N; ++col) {
+        // pivot: the largest |M[r][col]|, r >= col (lowest r on ties)
+        double v = (t >= col && t < N) ? fabs(M[t][col]) : -1.0;
+        int r = t;
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double v2 = __shfl_xor(v, o, 64);
+            const int r2 = __shfl_xor(r, o, 64);
+            if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
+        }
+        if (t == 0) piv_s = r;
+        __syncthreads();
+        const int pr = piv_s;
+        // uniform (every lane reads the same values); pr is outside col..N-1 only
+        // when the column holds NaNs, and is then not used as an index
+        if (pr < col || pr >= N || !(fabs(M[pr][col]) >= 1e-300)) { good = false; break; }
+        if (pr != col && t <= N) {
+            const double a = M[col][t], b = M[pr][t];
+            M[col][t] = b;
+            M[pr][t] = a;
+        }
+        __syncthreads();
+        const double f = (t < N && t != col) ? M[t][col] / M[col][col] : 0.0;
+        __syncthreads();
+        if (t < N && t != col)
+            for (int k = col; k <= N; ++k) M[t][k] -= f * M[col][k];
+        __syncthreads();
+    }
+    if (t < m) coef[t] = good ? M[t][N] / M[t][t] : 0.0;
+    if (t == 0) ok[0] = good ? 1 : 0;
+}
+
+__global__ void diis_extrap_kernel(int64_t n, int nstore, const double *coef, const int *ok, DiisPtrs v,
+                                   const double *F, double *Fx) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (!ok[0]) {
+        Fx[i] = F[i];
+        return;
+    }
+    double s = 0.0;
+    for (int k = 0; k < nstore; ++k) s += coef[k] * v.p[k][i];
+    Fx[i] = s;
+}
+
+void dev_diis_solve(int ndiis, int nstore, int slot, const double *dots, double *B, double *coef, int *ok,
+                    hipStream_t st) {
+    if (ndiis > DiisPtrs::MAX || nstore > ndiis || slot < 0 || slot >= ndiis) throw Error("dev_diis_solve: sizes");
+    diis_solve_kernel<<<1, 64, 0, st>>>(ndiis, nstore, slot, dots, B, coef, ok);
+    PF_HIP(hipGetLastError());
+}
+
+void dev_diis_extrap(int64_t n, int nstore, const double *coef, const int *ok, const DiisPtrs &v, const double *F,
+                     double *Fx, hipStream_t st) {
+    if (nstore < 1 || nstore > DiisPtrs::MAX) throw Error("dev_diis_extrap: 1..32 vectors");
+    diis_extrap_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, nstore, coef, ok, v, F, Fx);
+    PF_HIP(hipGetLastError());
+}
+
 // Single-workgroup reduction for the Jacobi solver's small helper sums.
 template <typename Fn>
 __global__ void __launch_bounds__(1024) reduce_kernel(int64_t n, Fn f, double *out) {

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -81,7 +82,20 @@ Basis build_basis_lines(const std::vector<std::string> &lines, const std::string
 Basis build_basis_blocks(const BasisBlocks &blocks, const std::vector<std::string> &species, const Molecule &mol,
                          bool renormalise);
 std::string resolve_basis_path(const std::string &name);
+// The parsed file at path, shared by every call (re-read when it changes).
+std::shared_ptr<const BasisBlocks> basis_blocks_cached(const std::string &path);
 double nuclear_repulsion(const Molecule &mol);
+
+// A device block from the process-wide pool of large scratch blocks: taken
+// with pool_acquire (at least `bytes`, on the current device) and given back
+// with pool_release once no pending work uses it.
+struct PooledBlock {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int dev = -1;
+};
+PooledBlock pool_acquire(size_t bytes);
+void pool_release(PooledBlock &b);
 
 // Device buffer with RAII.
 template <typename T>
