@@ -49,9 +49,18 @@ Input make_input(int natoms, const char *species, const int32_t *z, const double
 
 template <typename Fn>
 int guarded(Fn &&fn) {
+    // A call that failed part-way may have released buffers (into the block
+    // cache) with work on them still queued: drain the device before returning.
+    struct DrainOnError {
+        bool failed = true;
+        ~DrainOnError() {
+            if (failed) (void)hipDeviceSynchronize();
+        }
+    } drain;
     try {
         g_last_error.clear();
         fn();
+        drain.failed = false;
         return PF_OK;
     } catch (const Error &e) {
         g_last_error = e.what();

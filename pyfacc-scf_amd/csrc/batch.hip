@@ -435,7 +435,10 @@ BatchEngine::BatchEngine(std::vector<BatchMolecule> mols, const std::string &bas
 }
 
 BatchEngine::~BatchEngine() {
-    if (st_) (void)hipStreamDestroy(st_);
+    if (st_) {
+        (void)hipStreamSynchronize(st_);  // members go back to the block cache: nothing may be pending
+        (void)hipStreamDestroy(st_);
+    }
 }
 
 void BatchEngine::setup() {

@@ -97,6 +97,13 @@ struct PooledBlock {
 PooledBlock pool_acquire(size_t bytes);
 void pool_release(PooledBlock &b);
 
+// Device memory for DevBuf through a per-thread cache of released blocks
+// (by device and size class): the library's calls each use one stream and
+// synchronise it before returning, so a released block is reused only by the
+// same thread, ordered after its last use.  dev_free takes the requested size.
+void *dev_alloc(size_t bytes);
+void dev_free(void *p, size_t bytes);
+
 // Device buffer with RAII.
 template <typename T>
 struct DevBuf {
@@ -106,11 +113,11 @@ struct DevBuf {
     explicit DevBuf(size_t count) { alloc(count); }
     void alloc(size_t count) {
         release();
+        if (count) p = static_cast<T *>(dev_alloc(count * sizeof(T)));
         n = count;
-        if (count) PF_HIP(hipMalloc(&p, count * sizeof(T)));
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p, n * sizeof(T));
         p = nullptr;
         n = 0;
     }

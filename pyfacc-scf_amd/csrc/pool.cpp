@@ -8,6 +8,7 @@
 // is idle.  Blocks stay allocated until the process ends (at most POOL_KEEP
 // of them, per process; beyond that a released block is freed).
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -72,6 +73,79 @@ void pool_release(PooledBlock &b) {
         if (cur != b.dev) (void)hipSetDevice(cur);
     }
     b = PooledBlock{};
+}
+
+// ---- per-thread cache behind DevBuf
+//
+// Every call of the library issues its device work on one stream and
+// synchronises that stream before it returns.  A buffer released during a
+// call can therefore be reused by a later allocation of the same thread:
+// within the call the new owner's work is ordered after the old owner's on
+// the same stream, and across calls the old stream is idle.  Buffers are
+// kept per thread (never handed to another thread) and by exact size class,
+// so a repeated SCF reuses every buffer of the previous one.  At most
+// CACHE_MAX bytes stay cached per thread (beyond: hipFree); blocks over
+// CACHE_BLOCK_MAX (the batched integral store) are never cached.  Cached
+// memory is returned to the driver when the process ends.
+namespace {
+
+constexpr size_t CACHE_MAX = size_t(8) << 30;
+constexpr size_t CACHE_BLOCK_MAX = size_t(2) << 30;
+
+size_t size_class(size_t bytes) {
+    if (bytes <= 4096) return 4096;
+    if (bytes <= (size_t(1) << 20)) {  // powers of two up to 1 MiB
+        size_t c = 4096;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const size_t mib = size_t(1) << 20;  // then whole MiB
+    return (bytes + mib - 1) / mib * mib;
+}
+
+struct ThreadCache {
+    std::map<std::pair<int, size_t>, std::vector<void *>> free;  // (device, class) -> blocks
+    size_t bytes = 0;
+};
+ThreadCache &thread_cache() {
+    // heap object never destroyed: no hipFree during process teardown
+    static thread_local ThreadCache *c = new ThreadCache();
+    return *c;
+}
+
+}  // namespace
+
+void *dev_alloc(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    const size_t cls = size_class(bytes);
+    int dev = 0;
+    PF_HIP(hipGetDevice(&dev));
+    if (cls <= CACHE_BLOCK_MAX) {
+        auto &tc = thread_cache();
+        auto it = tc.free.find({dev, cls});
+        if (it != tc.free.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            tc.bytes -= cls;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    PF_HIP(hipMalloc(&p, cls));
+    return p;
+}
+
+void dev_free(void *p, size_t bytes) {
+    if (!p) return;
+    const size_t cls = size_class(bytes);
+    auto &tc = thread_cache();
+    int dev = 0;
+    if (cls <= CACHE_BLOCK_MAX && tc.bytes + cls <= CACHE_MAX && hipGetDevice(&dev) == hipSuccess) {
+        tc.free[{dev, cls}].push_back(p);
+        tc.bytes += cls;
+        return;
+    }
+    (void)hipFree(p);
 }
 
 }  // namespace pf
