@@ -18,6 +18,7 @@ namespace pf {
 struct PairSet {
     int la = 0, lb = 0;
     int n = 0, npad = 0, kp = 0;
+    double peps = 0.0;         // ket-primitive cut against the bra's bound (0: none)
     std::vector<int> sa, sb;   // host copies, sorted by Q descending
     std::vector<double> Q;
     DevBuf<int> d_sa, d_sb, d_oa, d_ob, d_kpn;

@@ -41,6 +41,7 @@ PairView<double> PairSet::view64() const {
     v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
     v.Q = d_Q.p; v.AB = d_AB.p; v.prim = d_prim.p;
     v.kpn = d_kpn.p;
+    v.peps = peps;
     return v;
 }
 PairView<float> PairSet::view32() const {
@@ -49,6 +50,7 @@ PairView<float> PairSet::view32() const {
     v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
     v.Q = f_Q.p; v.AB = f_AB.p; v.prim = f_prim.p;
     v.kpn = d_kpn.p;
+    v.peps = float(peps);
     return v;
 }
 

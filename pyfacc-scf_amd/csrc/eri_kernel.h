@@ -69,16 +69,21 @@ constexpr int jk_chunk() {
 // Contracted (ab|cd) block, eri[((ia*NB+ib)*NC+ic)*ND+id], of bra pair i
 // (uniform; its primitive data through bprim) and ket pair j (this lane).
 template <int LA, int LB, int LC, int LD, typename R>
-__device__ __forceinline__ void eri_block(const PairView<R> &B, const R *__restrict__ bprim, int i, R ABx, R ABy,
+__device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restrict__ bprim, int i, R ABx, R ABy,
                                           R ABz, const PairView<R> &K, int j, const R *lds_boys, R *eri) {
     using E = Eri<LA, LB, LC, LD, R>;
     R acc[E::NACC];
 #pragma unroll
     for (int k = 0; k < E::NACC; ++k) acc[k] = R(0);
-    // significant primitive pairs only (bra: uniform count; ket: this lane's)
+    // significant primitive pairs only (bra: uniform count; ket: this lane's,
+    // and of those only the ones whose bound times the bra's largest reaches
+    // the cut: ket primitives are sorted by bound, so the loop stops there)
     const int kpk = K.kpn[j], kpb = __builtin_amdgcn_readfirstlane(B.kpn[i]);
-    for (int u = 0; u < kpk; ++u) {
+    const R bmax = bprim[(size_t)PF_BND * B.npad + i], cut = B.peps;  // bra primitive 0 has the largest bound
+    int nu = 0;
+    for (int u = 0; u < kpk; ++u, ++nu) {
         const R *kq = K.prim + (size_t)u * PF_NFIELD * K.npad + j;
+        if (kq[PF_BND * K.npad] * bmax < cut) break;
         const R q = kq[PF_P * K.npad], Qx = kq[PF_PX * K.npad], Qy = kq[PF_PY * K.npad], Qz = kq[PF_PZ * K.npad];
         const R QCx = kq[PF_AX * K.npad], QCy = kq[PF_AY * K.npad], QCz = kq[PF_AZ * K.npad],
                 Kk = kq[PF_K * K.npad], o2q = kq[PF_O2P * K.npad];
@@ -92,6 +97,7 @@ __device__ __forceinline__ void eri_block(const PairView<R> &B, const R *__restr
     }
     const R CDx = K.AB[j], CDy = K.AB[K.npad + j], CDz = K.AB[2 * K.npad + j];
     E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, eri);
+    return nu * kpb;  // primitive quartets evaluated
 }
 
 // Densities a digest reads: Dj for J (the total density), Dk0/Dk1 for K.
@@ -252,14 +258,13 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
         if (j0 + lane >= nket) break;
         const int j = kbase + j0 + lane;
         ++counted;
-        if constexpr (COMPUTE) pcounted += (unsigned long long)K.kpn[j] * B.kpn[i];
         R eri[NINT];
         if constexpr (MODE == MODE_LOAD) {
             const R *src = g.store + g.store_off[i] + (size_t)j0 * NINT + lane;
 #pragma unroll
             for (int k = 0; k < NINT; ++k) eri[k] = src[k * 64];
         } else {
-            eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
+            pcounted += eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
         }
         if constexpr (MODE == MODE_STORE) {
             R *dst = g.store + g.store_off[i] + (size_t)j0 * NINT + lane;
@@ -368,7 +373,7 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
         const int wb = min(jend, jbeg + wave * per), we = min(jend, wb + per);
 
         const R Qi = B.Q[i];
-        const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i], kpn_i = B.kpn[i];
+        const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
         const R ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
         const R deg_ab = (a == b) ? R(0.5) : R(1);
         R dab[NA * NB], jab[NA * NB];
@@ -426,9 +431,8 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
             if (j < 0) continue;
             const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
             ++counted;
-            pcounted += (unsigned long long)K.kpn[j] * kpn_i;
             R eri[NINT];
-            eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
+            pcounted += eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
             double jcd[NCD];  // this quartet's J'_cd block
             const auto jadd = [&](int row, int col, R v) { jcd[(row - oc) * E::ND + (col - od)] = double(v); };

@@ -17,7 +17,12 @@ void launch_onee_batch(int nmol, int max_nsh, const Shell *d_shells, const int *
 
 // ---------------------------------------------------------------- pairs
 // Per primitive pair data, in this field order, stored structure-of-arrays.
-enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_O2P, PF_NFIELD };
+enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_O2P, PF_BND, PF_NFIELD };
+// A ket primitive pair is visited for a bra pair only when its bound PF_BND =
+// |K|/sqrt(q) times the bra's largest primitive bound reaches PRIM_EPS
+// (|(ab|cd)_prim| is at most the product of the two bounds times the
+// polynomial factors of the p-type recurrences).
+constexpr double PRIM_EPS = 1e-16;
 // PF_AX.. hold P - A (A = first shell of the pair); PF_K holds
 // sqrt(2) pi^{5/4} c_a c_b exp(-ab/p |AB|^2) / p, so that the quartet
 // prefactor is K_bra K_ket / sqrt(p + q); PF_O2P holds 1/(2p).
@@ -33,6 +38,7 @@ struct PairView {
     const R *AB = nullptr;                   // [3][npad]  A - B
     const R *prim = nullptr;                 // [kp][PF_NFIELD][npad], per pair by bound |K|/sqrt(p) descending
     const int *kpn = nullptr;                // [npad] primitive pairs that the ERI kernels visit
+    R peps = R(0);                           // ket-primitive cut against the bra's bound (0: none)
 };
 
 // Work description shared by all ERI/J/K kernels of one launch.

@@ -7,15 +7,13 @@
 // visit; a primitive quartet is at most K_ab K_cd / sqrt(p+q) times the
 // polynomial factors of the p-type recurrences, so the dropped ones are below
 // 1e-16 up to those factors; kref < 0 keeps every primitive), and writes the
-// fields P, P - A, K, 1/2p with stride npad.  Padding pairs and primitives get
+// fields P, P - A, K, 1/2p and the bound with stride npad.  Padding pairs and primitives get
 // a unit exponent and zero weight.
 #include "engine.h"
 
 namespace pf {
 
 namespace {
-
-constexpr double PRIM_EPS = 1e-16;
 
 __global__ void __launch_bounds__(256) build_pairs_kernel(const Shell *__restrict__ sh, const int *__restrict__ list,
                                                           int n, int npad, int kp, double kref, double *prim,
@@ -84,6 +82,7 @@ __global__ void __launch_bounds__(256) build_pairs_kernel(const Shell *__restric
         }
         col[PF_K * npad] = kfac * A.coef[x] * B.coef[y] * exp(-a * b / p * r2) / p;
         col[PF_O2P * npad] = 0.5 / p;
+        col[PF_BND * npad] = bnd[u];
     }
     for (int u = no; u < kp; ++u) pad_prim(u);
     kpn[i] = cnt;
@@ -116,6 +115,7 @@ void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shel
                   const std::vector<std::pair<int, int>> &list, double kref, double &bound_max, bool want32,
                   hipStream_t st) {
     ps.n = (int)list.size();
+    ps.peps = kref < 0.0 ? 0.0 : PRIM_EPS;
     ps.npad = round64(std::max(ps.n, 1));
     int kp = 1;
     for (auto &pr : list) kp = std::max(kp, sh[pr.first].np * sh[pr.second].np);
