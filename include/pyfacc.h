@@ -119,6 +119,20 @@ int pf_one_electron(int natoms, const char *species, const int32_t *z, const dou
 int pf_eri_packed(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
                   int renormalize, double *packed);
 
+/* ERI blocks of shell quartets in Libint's layout: quartet q is
+ * (quartets[4q] quartets[4q+1] | quartets[4q+2] quartets[4q+3]), 0-based shell
+ * indices in the order of pf_basis_info, any orientation; its block holds
+ * n1 n2 n3 n4 values (n = 1 for s, 3 for p), index ((i1 n2 + i2) n3 + i3) n4 + i4
+ * over the shells' Cartesian components (x, y, z), over contracted functions
+ * normalised when renormalize = 1 (Libint's convention) or with primitive
+ * norms only (0).  Blocks are concatenated in list order; offsets (may be
+ * NULL) receives nquart + 1 entries, block q at out + offsets[q].  Replaces
+ * what compute_ERI_libint receives from Libint per quartet before scattering
+ * it (libint_integrals.f90:111-139); every primitive quartet is included. */
+int pf_eri_shell_quartets(int natoms, const char *species, const int32_t *z, const double *coords,
+                          const char *basis_name, int renormalize, int nquart, const int32_t *quartets, double *out,
+                          int64_t *offsets);
+
 /* J[D] (summed over densities) and K[D_k] per density, for ndens = 1 or 2
  * host densities (ndens x nbf x nbf).  eri_mode as in pf_options; precision
  * PF_PREC_MIXED evaluates in FP32.  schwarz <= 0 selects 1e-14. */

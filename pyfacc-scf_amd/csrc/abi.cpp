@@ -153,6 +153,21 @@ int pf_eri_packed(int natoms, const char *species, const int32_t *z, const doubl
     });
 }
 
+int pf_eri_shell_quartets(int natoms, const char *species, const int32_t *z, const double *coords,
+                          const char *basis_name, int renormalize, int nquart, const int32_t *quartets, double *out,
+                          int64_t *offsets) {
+    return guarded([&] {
+        if (nquart < 0) throw Error("nquart must be >= 0");
+        if (nquart > 0 && (!quartets || !out)) throw Error("null quartets or out");
+        Input in = make_input(natoms, species, z, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), renormalize != 0, 0.0);
+        std::vector<long long> off((size_t)nquart + 1);
+        eng.eri_quartets(nquart, quartets, out, off.data());
+        if (offsets)
+            for (int q = 0; q <= nquart; ++q) offsets[q] = off[q];
+    });
+}
+
 int pf_jk(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
           int renormalize, int ndens, const double *D, double *J, double *K, int eri_mode, int precision,
           double schwarz) {

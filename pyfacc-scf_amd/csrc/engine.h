@@ -70,6 +70,10 @@ class Engine {
     // call; two-electron quartets screened at tau on Q Q max|Gamma|.
     // terms (optional, 5 x natoms x 3): nuclear repulsion, kinetic (D dT),
     // nuclear attraction (D dV), overlap (-W dS), two-electron.
+    // ERI blocks of shell quartets quartets[4q..4q+3] (shell indices) in
+    // Libint's layout, concatenated into out (host); offsets[q] (host, nq + 1
+    // entries, may be null) is where block q starts.
+    void eri_quartets(int nq, const int *quartets, double *out, long long *offsets);
     void gradient(double *grad_out, double tau, double *terms = nullptr);
 
     // Fock-build statistics accumulated over an SCF, index precision*6 + class

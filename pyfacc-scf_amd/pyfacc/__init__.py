@@ -20,6 +20,7 @@ from ._native import (  # noqa: F401
     device_count,
     eigh,
     eri_packed,
+    eri_quartets,
     gemm,
     jk,
     load_library,

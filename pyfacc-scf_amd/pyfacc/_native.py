@@ -50,6 +50,8 @@ _SIGS = {
                            _P(_i), _pd, _pd]),
     "pf_one_electron": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd, _pd, _pd]),
     "pf_eri_packed": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd]),
+    "pf_eri_shell_quartets": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pi32, _pd,
+                                   _P(ctypes.c_int64)]),
     "pf_jk": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pd, _pd, _pd, _i, _i, _d]),
     "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
     "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i), _P(ctypes.c_ulonglong)]),
@@ -159,6 +161,27 @@ def eri_packed(species, coords_bohr, basis="sto-3g", renormalize=True):
     m = _Mol(species, coords_bohr)
     _check(load_library().pf_eri_packed(*m.args(), basis.encode(), int(renormalize), _ptr(out)))
     return out
+
+
+def eri_quartets(species, coords_bohr, quartets, basis="sto-3g", renormalize=True):
+    """ERI blocks of shell quartets in Libint's layout (``pf_eri_shell_quartets``).
+
+    quartets: (nq, 4) shell indices (order of ``basis_info``), any orientation.
+    Returns a list of arrays shaped (n1, n2, n3, n4)."""
+    qs = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
+    info = basis_info(species, coords_bohr, basis, renormalize)
+    nc = (np.asarray(info["l"]) + 1) * (np.asarray(info["l"]) + 2) // 2
+    if qs.size and (qs.min() < 0 or qs.max() >= len(nc)):
+        raise PyfaccError("eri_quartets: shell index out of range")
+    dims = nc[qs] if qs.size else np.zeros((0, 4), dtype=np.int64)
+    sizes = dims.prod(axis=1) if qs.size else np.zeros(0, dtype=np.int64)
+    out = np.zeros(max(int(sizes.sum()), 1))
+    off = np.zeros(len(qs) + 1, dtype=np.int64)
+    m = _Mol(species, coords_bohr)
+    _check(load_library().pf_eri_shell_quartets(*m.args(), basis.encode(), int(renormalize), len(qs),
+                                                qs.ctypes.data_as(_pi32), _ptr(out),
+                                                off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    return [out[off[q]:off[q + 1]].reshape(tuple(dims[q])) for q in range(len(qs))]
 
 
 def jk(species, coords_bohr, D, basis="sto-3g", renormalize=True, stored=False, fp32=False, schwarz=0.0):
