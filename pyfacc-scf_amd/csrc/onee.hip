@@ -2,9 +2,10 @@
 //
 // Replaces the reference's three OpenACC shell-pair loops S_overlap
 // (overlap.f90:240-340), T_kinetic (kinetic.f90:117-214) and V_nuclear
-// (nuclear.f90:202-328) with one fused kernel: one thread per shell pair
-// (a >= b), all primitive pairs and all nuclei in registers, the Boys table
-// staged once per workgroup in LDS.  No per-atom V scratch (the reference
+// (nuclear.f90:202-328) with one fused kernel: per shell pair (a >= b) all
+// primitive pairs in registers, the nuclei split over 16 lanes (one thread
+// per pair in the batched form, whose many small molecules fill the chip), the
+// Boys table staged once per workgroup in LDS.  No per-atom V scratch (the reference
 // holds natoms x nbf^2 doubles) and no host-side summation.
 //
 //   overlap 1-D OS:  S_{i+1,j} = PA S_ij + (i S_{i-1,j} + j S_{i,j-1}) / 2p
@@ -24,7 +25,8 @@ constexpr int ONEE_NORD = 2 + BOYS_TAYLOR;
 
 template <int LA, int LB>
 __device__ void onee_block(const Shell &A, const Shell &B, int natoms, const double *__restrict__ axyz,
-                           const int *__restrict__ Z, const double *tab, double *S, double *T, double *V) {
+                           const int *__restrict__ Z, const double *tab, double *S, double *T, double *V, int c0,
+                           int cs) {
     constexpr int NA = nc(LA), NB = nc(LB);
     constexpr int LT = LA + LB;
     double s_acc[NA * NB], t_acc[NA * NB], v_acc[NA * NB];
@@ -88,7 +90,7 @@ __device__ void onee_block(const Shell &A, const Shell &B, int natoms, const dou
             }
             // nuclear attraction
             const double vpre = cc * kab * 2.0 * M_PI * ip;
-            for (int c = 0; c < natoms; ++c) {
+            for (int c = c0; c < natoms; c += cs) {  // this thread's nuclei
                 double PC[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) PC[d] = P[d] - axyz[3 * c + d];
@@ -139,25 +141,30 @@ __device__ void onee_block(const Shell &A, const Shell &B, int natoms, const dou
     }
 }
 
-// One shell pair (a >= b, pair index pid) of one molecule: S, T, V blocks
-// written to both triangles.
-__device__ __forceinline__ void onee_pair(const Shell *__restrict__ shells, int pid, int nbf, int natoms,
-                                          const double *__restrict__ axyz, const int *__restrict__ Z,
-                                          const double *lds_tab, double *Smat, double *Tmat, double *Vmat) {
-    // pair index -> (a, b), a >= b
-    int a = int((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
+// Shell pair pid (a >= b) -> (a, b).
+__device__ __forceinline__ void onee_pair_index(int pid, int &a, int &b) {
+    a = int((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
     while (a * (a + 1) / 2 > pid) --a;
     while ((a + 1) * (a + 2) / 2 <= pid) ++a;
-    const int b = pid - a * (a + 1) / 2;
-    const Shell &A = shells[a];
-    const Shell &B = shells[b];
-    double s[9], t[9], v[9];
+    b = pid - a * (a + 1) / 2;
+}
+
+// S, T and the nuclei c0, c0 + cs, ... of V for shell pair (A, B) (s[k], t[k],
+// v[k], k = ia * nb + ib).
+__device__ __forceinline__ void onee_pair_blocks(const Shell &A, const Shell &B, int natoms,
+                                                 const double *__restrict__ axyz, const int *__restrict__ Z,
+                                                 const double *lds_tab, double *s, double *t, double *v, int c0,
+                                                 int cs) {
     const int la = A.l, lb = B.l;
-    if (la == 0 && lb == 0) onee_block<0, 0>(A, B, natoms, axyz, Z, lds_tab, s, t, v);
-    else if (la == 1 && lb == 0) onee_block<1, 0>(A, B, natoms, axyz, Z, lds_tab, s, t, v);
-    else if (la == 0 && lb == 1) onee_block<0, 1>(A, B, natoms, axyz, Z, lds_tab, s, t, v);
-    else onee_block<1, 1>(A, B, natoms, axyz, Z, lds_tab, s, t, v);
-    const int na = nc(la), nb = nc(lb);
+    if (la == 0 && lb == 0) onee_block<0, 0>(A, B, natoms, axyz, Z, lds_tab, s, t, v, c0, cs);
+    else if (la == 1 && lb == 0) onee_block<1, 0>(A, B, natoms, axyz, Z, lds_tab, s, t, v, c0, cs);
+    else if (la == 0 && lb == 1) onee_block<0, 1>(A, B, natoms, axyz, Z, lds_tab, s, t, v, c0, cs);
+    else onee_block<1, 1>(A, B, natoms, axyz, Z, lds_tab, s, t, v, c0, cs);
+}
+
+__device__ __forceinline__ void onee_store(const Shell &A, const Shell &B, int nbf, const double *s, const double *t,
+                                           const double *v, double *Smat, double *Tmat, double *Vmat) {
+    const int na = nc(A.l), nb = nc(B.l);
     for (int ia = 0; ia < na; ++ia)
         for (int ib = 0; ib < nb; ++ib) {
             const size_t mu = A.bf + ia, nu = B.bf + ib, k = ia * nb + ib;
@@ -165,6 +172,18 @@ __device__ __forceinline__ void onee_pair(const Shell *__restrict__ shells, int 
             Tmat[mu * nbf + nu] = t[k]; Tmat[nu * nbf + mu] = t[k];
             Vmat[mu * nbf + nu] = v[k]; Vmat[nu * nbf + mu] = v[k];
         }
+}
+
+// One shell pair (a >= b, pair index pid) of one molecule by one thread: S, T,
+// V blocks written to both triangles.
+__device__ __forceinline__ void onee_pair(const Shell *__restrict__ shells, int pid, int nbf, int natoms,
+                                          const double *__restrict__ axyz, const int *__restrict__ Z,
+                                          const double *lds_tab, double *Smat, double *Tmat, double *Vmat) {
+    int a, b;
+    onee_pair_index(pid, a, b);
+    double s[9], t[9], v[9];
+    onee_pair_blocks(shells[a], shells[b], natoms, axyz, Z, lds_tab, s, t, v, 0, 1);
+    onee_store(shells[a], shells[b], nbf, s, t, v, Smat, Tmat, Vmat);
 }
 
 __device__ __forceinline__ void onee_stage_table(const double *__restrict__ gtab, double *lds_tab) {
@@ -176,15 +195,33 @@ __device__ __forceinline__ void onee_stage_table(const double *__restrict__ gtab
     __syncthreads();
 }
 
+// One molecule: ONEE_LPP lanes per shell pair split the nuclear sum (V from a
+// sub-wavefront shuffle reduction), so that a large molecule's tens of
+// thousands of pairs fill the chip; S and T are computed alike by the lanes.
+constexpr int ONEE_LPP = 16;
 __global__ void __launch_bounds__(256) onee_kernel(const Shell *__restrict__ shells, int nsh, int nbf, int natoms,
                                                    const double *__restrict__ axyz, const int *__restrict__ Z,
                                                    const double *__restrict__ gtab, double *Smat, double *Tmat,
                                                    double *Vmat) {
     extern __shared__ double lds_tab[];
     onee_stage_table(gtab, lds_tab);
-    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid >= nsh * (nsh + 1) / 2) return;
-    onee_pair(shells, pid, nbf, natoms, axyz, Z, lds_tab, Smat, Tmat, Vmat);
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pid = g / ONEE_LPP, sub = g % ONEE_LPP;
+    const bool valid = pid < nsh * (nsh + 1) / 2;
+    double s[9], t[9], v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s[k] = t[k] = v[k] = 0.0;
+    int a = 0, b = 0;
+    if (valid) {
+        onee_pair_index(pid, a, b);
+        onee_pair_blocks(shells[a], shells[b], natoms, axyz, Z, lds_tab, s, t, v, sub, ONEE_LPP);
+    }
+    // every lane takes part (groups of ONEE_LPP lanes lie within one wavefront)
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int o = ONEE_LPP / 2; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if (valid && sub == 0) onee_store(shells[a], shells[b], nbf, s, t, v, Smat, Tmat, Vmat);
 }
 
 // Many molecules in one launch: blockIdx.y = molecule m, whose shells start at
@@ -218,8 +255,8 @@ void launch_onee_batch(int nmol, int max_nsh, const Shell *d_shells, const int *
 
 void launch_onee(const Shell *d_shells, int nsh, int nbf, int natoms, const double *d_axyz, const int *d_Z,
                  const double *d_tab, double *S, double *T, double *V, hipStream_t st) {
-    const int npair = nsh * (nsh + 1) / 2;
-    const int blocks = (npair + 255) / 256;
+    const long long nthreads = (long long)nsh * (nsh + 1) / 2 * ONEE_LPP;
+    const int blocks = (int)((nthreads + 255) / 256);
     const size_t lds = sizeof(double) * BOYS_NGRID * ONEE_NORD;
     onee_kernel<<<blocks, 256, lds, st>>>(d_shells, nsh, nbf, natoms, d_axyz, d_Z, d_tab, S, T, V);
     PF_HIP(hipGetLastError());
