@@ -658,19 +658,30 @@ bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_
 static int sp2_tiles(int n) { return ((n + 31) / 32) * ((n + 31) / 32); }
 size_t sp2_state_size(int n) { return 8 + 6 * (size_t)sp2_tiles(n) + 2 * (size_t)n; }
 
-// X0 = (hi I - F) / (hi - lo) on 32x32 tiles, with slot-0 partials and records.
+// Tile t of the upper triangle of the nt x nt tile grid (row-major) -> (ty, tx), ty <= tx.
+__device__ __forceinline__ void tri_tile(int t, int nt, int &ty, int &tx) {
+    ty = 0;
+    while (t >= nt - ty) { t -= nt - ty; ++ty; }
+    tx = ty + t;
+}
+
+// X0 = (hi I - F) / (hi - lo) on the upper-triangle 32x32 tiles (each written to
+// both halves, so X0 is exactly symmetric), with slot-0 partials and records.
 __global__ void __launch_bounds__(256) sp2_init_kernel(int n, const double *F, double *state, double *X) {
     __shared__ double sh[4];
-    const int nt = (n + 31) / 32, tile = blockIdx.y * nt + blockIdx.x, ntile = nt * nt;
-    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int nt = (n + 31) / 32, tile = blockIdx.x, ntile = nt * (nt + 1) / 2;
+    int ty, tx;
+    tri_tile(tile, nt, ty, tx);
+    const int r0 = ty * 32, c0 = tx * 32;
     const double lo = state[0], hi = state[1], sc = 1.0 / (hi - lo);
     double t1 = 0.0, t2 = 0.0;
     for (int e = threadIdx.x; e < 1024; e += 256) {
         const int r = r0 + (e >> 5), c = c0 + (e & 31);
-        if (r < n && c < n) {
+        if (r < n && c < n && (ty != tx || r <= c)) {
             const double v = ((r == c ? hi : 0.0) - F[(size_t)r * n + c]) * sc;
             X[(size_t)r * n + c] = v;
-            t2 += v * v;
+            X[(size_t)c * n + r] = v;
+            t2 += r == c ? v * v : 2.0 * v * v;
             if (r == c) t1 += v;
         }
     }
@@ -688,12 +699,14 @@ __global__ void __launch_bounds__(256) sp2_init_kernel(int n, const double *F, d
 // |t2 - nocc| < |2 t1 - t2 - nocc|, else 2X - X^2; once the idempotency error
 // |t1 - t2| is below tol (or stops shrinking below 1e-8, after step 2) the
 // step copies X and the done flag is raised.  X' partials and records go to
-// slot (k+1) % 3.
-__global__ void __launch_bounds__(64 * GW) sp2_step_kernel(int n, double nocc, double tol, int k, const double *X,
+// slot (k+1) % 3.  X' is symmetric: only the upper-triangle tiles are
+// computed, each written to both halves.
+constexpr int SW = GW;  // wavefronts per tile of the purification step
+__global__ void __launch_bounds__(64 * SW) sp2_step_kernel(int n, double nocc, double tol, int k, const double *X,
                                                         double *Xo, double *state) {
-    __shared__ double red[GW][16][64];
-    __shared__ double sh[GW];
-    const int nt = (n + 31) / 32, tile = blockIdx.y * nt + blockIdx.x, ntile = nt * nt;
+    __shared__ double red[SW][16][64];
+    __shared__ double sh[SW];
+    const int nt = (n + 31) / 32, tile = blockIdx.x, ntile = nt * (nt + 1) / 2;
     const int s0 = k % 3, s1 = (k + 1) % 3;
     // record [3 + 2s]: steps since convergence (0 = not converged).  The first
     // two converged steps copy X into both buffers; later ones do nothing.
@@ -707,39 +720,42 @@ __global__ void __launch_bounds__(64 * GW) sp2_step_kernel(int n, double nocc, d
     }
     const double *P = state + 8;
     double t1 = 0.0, t2 = 0.0;
-    for (int b = threadIdx.x; b < ntile; b += 64 * GW) {  // fixed order: deterministic sums
+    for (int b = threadIdx.x; b < ntile; b += 64 * SW) {  // fixed order: deterministic sums
         t1 += P[(2 * s0 + 0) * ntile + b];
         t2 += P[(2 * s0 + 1) * ntile + b];
     }
-    t1 = block_sum<GW>(t1, sh);
-    t2 = block_sum<GW>(t2, sh);
+    t1 = block_sum<SW>(t1, sh);
+    t2 = block_sum<SW>(t2, sh);
     const double err = fabs(t1 - t2), prev = state[2 + 2 * s0];
     bool done = age != 0.0;
     if (!done && k > 2 && (err < tol || (err < 1e-8 && err >= prev))) done = true;
-    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    int ty, tx;
+    tri_tile(tile, nt, ty, tx);
+    const int r0 = ty * 32, c0 = tx * 32;
     const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    if (!done) mfma_tile_w<GW, 4>(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
+    if (!done) mfma_tile_w<SW, 4>(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
     double u1 = 0.0, u2 = 0.0;
-    for (int idx = threadIdx.x; idx < 1024; idx += 64 * GW) {
+    for (int idx = threadIdx.x; idx < 1024; idx += 64 * SW) {
         const int reg = idx >> 6, ln = idx & 63;
         int gr, gc;
         tile_pos(idx, r0, c0, gr, gc);
-        if (gr < n && gc < n) {
+        if (gr < n && gc < n && (ty != tx || gr <= gc)) {
             const size_t o = (size_t)gr * n + gc;
             double v;
             if (done) {
                 v = X[o];
             } else {
-                const double x2 = red_sum<GW>(red, reg, ln);
+                const double x2 = red_sum<SW>(red, reg, ln);
                 v = square ? x2 : 2.0 * X[o] - x2;
             }
             Xo[o] = v;
-            u2 += v * v;
+            Xo[(size_t)gc * n + gr] = v;
+            u2 += gr == gc ? v * v : 2.0 * v * v;
             if (gr == gc) u1 += v;
         }
     }
-    u1 = block_sum<GW>(u1, sh);
-    u2 = block_sum<GW>(u2, sh);
+    u1 = block_sum<SW>(u1, sh);
+    u2 = block_sum<SW>(u2, sh);
     if (threadIdx.x == 0) {
         state[8 + (2 * s1 + 0) * ntile + tile] = u1;
         state[8 + (2 * s1 + 1) * ntile + tile] = u2;
@@ -753,7 +769,7 @@ __global__ void __launch_bounds__(64 * GW) sp2_step_kernel(int n, double nocc, d
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
                 int max_iter, int first_check) {
     const int nt = (n + 31) / 32;
-    const dim3 tiles(nt, nt);
+    const dim3 tiles(nt * (nt + 1) / 2);  // upper-triangle tiles
     double *rows = state + 8 + 6 * (size_t)sp2_tiles(n);
     gershgorin_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, Fp, rows, rows + n);
     gershgorin_bounds_kernel<<<1, 1024, 0, st>>>(n, rows, rows + n, state);
@@ -762,7 +778,7 @@ int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, d
     double *cur = X, *nxt = work;
     int used = max_iter, check = std::max(1, std::min(first_check, max_iter));
     for (int it = 0; it < max_iter; ++it) {
-        sp2_step_kernel<<<tiles, 64 * GW, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
+        sp2_step_kernel<<<tiles, 64 * SW, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
         std::swap(cur, nxt);
         if (it + 1 == check) {
             // record written by step it: steps since convergence (0: not yet)
