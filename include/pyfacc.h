@@ -161,9 +161,10 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
 /* Fock-build profile of the last pf_scf call on this thread, summed over its
  * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,
  * 1 FP32; classes (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)):
- * quartets that passed screening, kernel milliseconds (HIP events on the
- * engine's stream), kernel launches and primitive quartets evaluated (the
- * significant primitive pairs of bra times ket, summed over quartets);
+ * quartets that passed screening, milliseconds of the class's main ERI
+ * kernel (HIP events on the engine's stream right around it: the screening
+ * count/scan and J'_cd reduction helpers are not included), kernel launches
+ * and primitive quartets evaluated (summed over quartets);
  * class_l[24] gives per class the angular momenta (la lb | lc ld) in the
  * orientation launched.  Any output may be NULL. */
 int pf_scf_profile(unsigned long long *quartets, double *ms, int *launches, int *class_l,

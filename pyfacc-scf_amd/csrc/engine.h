@@ -77,8 +77,9 @@ class Engine {
     void gradient(double *grad_out, double tau, double *terms = nullptr);
 
     // Fock-build statistics accumulated over an SCF, index precision*6 + class
-    // (precision 0 = FP64, 1 = FP32): quartets, kernel milliseconds (HIP events
-    // on the engine stream, filled when profile_classes is set) and launches.
+    // (precision 0 = FP64, 1 = FP32): quartets, milliseconds of the main ERI
+    // kernel (HIP events on the engine stream right around it, filled when
+    // profile_classes is set) and launches.
     std::array<unsigned long long, 12> prof_quartets{};
     std::array<unsigned long long, 12> prof_prims{};  // primitive quartets evaluated
     std::array<double, 12> prof_ms{};
@@ -126,7 +127,7 @@ class Engine {
     DevBuf<unsigned long long> count_;  // [precision][quartets 6, primitive quartets 6]
     struct PendingBuild {
         int pbase;
-        hipEvent_t ev[7];
+        hipEvent_t ev[12];  // per class: before and after its main ERI kernel
     };
     std::vector<PendingBuild> pending_;  // builds whose class timings are not yet read
     std::array<DevBuf<int>, 6> plan_nket_, plan_off_, plan_work_;  // direct-mode work plans per class

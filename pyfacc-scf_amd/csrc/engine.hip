@@ -272,7 +272,7 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
     PendingBuild pb{pbase, {}};
     hipEvent_t *ev = pb.ev;
     if (profile_classes)
-        for (int e = 0; e < 7; ++e) PF_HIP(hipEventCreate(&ev[e]));
+        for (int e = 0; e < 12; ++e) PF_HIP(hipEventCreate(&ev[e]));
     for (size_t q = 0; q < classes_.size(); ++q) {
         const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
         if constexpr (std::is_same<R, double>::value) { a.bra = B.view64(); a.ket = K.view64(); }
@@ -287,10 +287,10 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
         a.pcount = count_.p + 2 * pbase + 6 + classes_[q].slot;
         a.nket = plan_nket_[q].p; a.item_off = plan_off_[q].p; a.work = plan_work_[q].p;
         a.joff = plan_joff_[q].p; a.jbuf = static_cast<double *>(jbuf_[q].p); a.jcap = jcap_[q]; a.jhi = plan_jhi_[q].p;
-        if (profile_classes) PF_HIP(hipEventRecord(ev[q], st_));
+        a.ev0 = profile_classes ? ev[2 * q] : nullptr;
+        a.ev1 = profile_classes ? ev[2 * q + 1] : nullptr;
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
-    if (profile_classes) PF_HIP(hipEventRecord(ev[6], st_));
     dev_assemble_g<double>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
     for (int q = 0; q < 6; ++q) prof_launches[pbase + q] += 1;
     if (profile_classes) pending_.push_back(pb);
@@ -312,7 +312,7 @@ unsigned long long Engine::collect_profile() {
     for (auto &b : pending_) {
         for (int q = 0; q < 6; ++q) {
             float ms = 0.f;
-            PF_HIP(hipEventElapsedTime(&ms, b.ev[q], b.ev[q + 1]));
+            PF_HIP(hipEventElapsedTime(&ms, b.ev[2 * q], b.ev[2 * q + 1]));
             prof_ms[b.pbase + classes_[q].slot] += ms;
         }
         for (auto &e : b.ev) (void)hipEventDestroy(e);

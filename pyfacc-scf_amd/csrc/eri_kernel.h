@@ -608,7 +608,9 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     const int grid = std::max(1, per) * std::max(1, ncu);  // persistent: as many workgroups as fit at once
     jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
     jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);
+    if (a.ev0) PF_HIP(hipEventRecord(a.ev0, st));
     kern<<<grid, 64 * W, dyn, st>>>(a, a.bra.prim, a.item_off);
+    if (a.ev1) PF_HIP(hipEventRecord(a.ev1, st));
     if (a.jbuf) {
         constexpr int NCD = nc(LC) * nc(LD);
         const long long nt = (long long)a.ket.n * NCD;
@@ -619,10 +621,7 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
-void launch_nk_mode(int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
-    if (args.bra.n == 0) return;
-    constexpr int W = jk_waves<LA, LB, LC, LD>();
-    const dim3 grid((args.bra.n + W - 1) / W), block(64 * W);
+void launch_nk_stored(int nk, int mode, const JKArgs<R> &args, dim3 grid, dim3 block, hipStream_t st) {
     if constexpr (std::is_same<R, double>::value) {
         if (mode == MODE_STORE) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_STORE><<<grid, block, 0, st>>>(args, args.bra.prim); return; }
         if (mode == MODE_PACKED) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_PACKED><<<grid, block, 0, st>>>(args, args.bra.prim); return; }
@@ -632,7 +631,25 @@ void launch_nk_mode(int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
             return;
         }
     }
-    if (mode != MODE_DIRECT) throw Error("jk: mode not available in this precision");
+    throw Error("jk: mode not available in this precision");
+}
+
+
+template <int LA, int LB, int LC, int LD, typename R>
+void launch_nk_mode(int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
+    if (args.bra.n == 0) {  // nothing to launch; the events still bracket an (empty) interval
+        if (args.ev0) PF_HIP(hipEventRecord(args.ev0, st));
+        if (args.ev1) PF_HIP(hipEventRecord(args.ev1, st));
+        return;
+    }
+    constexpr int W = jk_waves<LA, LB, LC, LD>();
+    const dim3 grid((args.bra.n + W - 1) / W), block(64 * W);
+    if (mode != MODE_DIRECT) {  // the stored-integral kernels are one launch: time it whole
+        if (args.ev0) PF_HIP(hipEventRecord(args.ev0, st));
+        launch_nk_stored<LA, LB, LC, LD, R>(nk, mode, args, grid, block, st);
+        if (args.ev1) PF_HIP(hipEventRecord(args.ev1, st));
+        return;
+    }
     if (nk == 1) launch_direct<LA, LB, LC, LD, R, 1>(args, st);
     else launch_direct<LA, LB, LC, LD, R, 2>(args, st);
 }
