@@ -116,3 +116,30 @@ def test_gradient_terms_vs_oracle_finite_differences(gpu_lib, name, na, nb):
         err = np.abs(r["terms"][k] - fd[k]).max()
         assert err < 1e-6, (names[k], err, r["terms"][k], fd[k])
     assert np.abs(g - fd.sum(axis=0)).max() < 1e-6
+
+
+# The differentiation workload: the seeded (H2O)32 STO-3G cluster, 224 basis
+# functions, 288 gradient components.
+def test_gradient_h2o32_vs_finite_differences(gpu_lib):
+    from pyfacc.grad import scf_gradient
+    sp, xyz = load_xyz("h2o_32.xyz")
+    e, g, r = scf_gradient(sp, xyz, 160, 160, eps=1e-11, max_iter=200)
+    assert r["converged"] and g.shape == (96, 3)
+    assert np.abs(g.sum(axis=0)).max() < 1e-7  # translational invariance over all 288 components
+    # a spread sample of components: first, middle and last molecules, O and H
+    coords = [(0, 0), (1, 1), (2, 2), (46, 0), (47, 2), (48, 1), (93, 2), (94, 0), (95, 1)]
+    fd = _fd(sp, xyz, 160, 160, coords)
+    an = np.array([g[a, x] for a, x in coords])
+    assert np.abs(an - fd).max() < 1e-6
+
+
+@pytest.mark.parametrize("na,nb", [(160, 160), (160, 159)])  # RHF, and the (H2O)32+ doublet (UHF)
+def test_gradient_h2o32_mixed_forward_matches_fp64(gpu_lib, na, nb):
+    from pyfacc.grad import scf_gradient
+    sp, xyz = load_xyz("h2o_32.xyz")
+    e64, g64, r64 = scf_gradient(sp, xyz, na, nb, eps=1e-10, max_iter=200)
+    emx, gmx, rmx = scf_gradient(sp, xyz, na, nb, eps=1e-10, max_iter=200, precision=1)
+    assert r64["converged"] and rmx["converged"]
+    assert abs(emx - e64) < 1e-6
+    assert np.abs(g64.sum(axis=0)).max() < 1e-7
+    assert np.abs(g64 - gmx).max() < 1e-6
