@@ -149,8 +149,8 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
 /* SCF (as pf_scf) followed by the analytic nuclear gradient dE/dR at the
  * converged density: grad[natoms*3] in Eh/bohr (one-electron, two-electron
  * and nuclear-repulsion terms; W = 1/2 D F D, RHF, or sum_s D_s F_s D_s, UHF).
- * The reference has no gradients; this is the north star's differentiation
- * extension, bound to torch.autograd by pyfacc.grad.  D_out may be NULL;
+ * The reference has no gradients; this extension is bound to torch.autograd
+ * by pyfacc.grad.  D_out may be NULL;
  * terms (may be NULL) receives 5 x natoms x 3 contributions: nuclear
  * repulsion, kinetic (D dT), nuclear attraction (D dV), overlap (-W dS),
  * two-electron. */

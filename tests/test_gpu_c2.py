@@ -1,10 +1,10 @@
-"""C2-size parity: the seeded (H2O)16 cluster (112 basis functions).
+"""Parity on the seeded (H2O)16 cluster (112 basis functions).
 
 * all 20 024 956 unique ERIs (STO-3G) from the GPU against the oracle, 1e-12;
 * GPU J/K (direct and stored) against the oracle's J/K from its packed ERIs;
 * RHF (H2O)16 and UHF (H2O)15OH energies against the oracle, 1e-9 Eh;
 * STO-6G (contraction depth 6) on a 4-water cut of the same cluster.
-The oracle runs on the host cores of the GPU box (OpenMP).
+The oracle runs on the host cores (OpenMP).
 """
 import numpy as np
 import pytest

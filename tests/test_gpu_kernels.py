@@ -1,6 +1,6 @@
 """GPU parity of the individual kernels against the oracle / library routines.
 
-Tolerances: 1e-12 absolute for FP64 integrals (north-star integral parity),
+Tolerances: 1e-12 absolute for FP64 integrals,
 1e-13 relative for Boys, 1e-11 for dense FP64 linear algebra.
 """
 import numpy as np
