@@ -52,6 +52,7 @@ enum { PF_OK = 0, PF_ERR_INPUT = 1, PF_ERR_BASIS = 2, PF_ERR_DEVICE = 3, PF_ERR_
 enum { PF_PREC_FP64 = 0, PF_PREC_MIXED = 1 };
 enum { PF_ERI_DIRECT = 0, PF_ERI_STORED = 1 };
 enum { PF_DIAG_SP2 = 0, PF_DIAG_JACOBI = 1 };
+enum { PF_GUESS_CORE = 0, PF_GUESS_SAD = 1 };
 
 typedef struct pf_options {
     double eps;           /* convergence threshold on |dE| and ||dD||_F (rscf.f90:152-155) */
@@ -70,6 +71,8 @@ typedef struct pf_options {
     int rebuild_every;    /* incremental FP64 builds: full rebuild after this many increments (0: never) */
     double schwarz_fp32;  /* mixed precision: screening threshold (on Q Q max|dD|; 0: schwarz) of the
                              FP32 incremental builds before the full FP64 rebuild that replaces their sum */
+    int guess;            /* PF_GUESS_CORE: core Hamiltonian (rscf.f90:113-114, the reference's); PF_GUESS_SAD:
+                             superposition of spherically averaged atomic UHF densities */
 } pf_options;
 
 typedef struct pf_result {
@@ -94,7 +97,7 @@ typedef struct pf_result {
 
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
- * rebuild_every 0, schwarz_fp32 1e-6. */
+ * rebuild_every 0, schwarz_fp32 1e-6, core guess. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

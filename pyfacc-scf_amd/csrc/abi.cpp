@@ -104,6 +104,7 @@ void pf_default_options(pf_options *o) {
     o->schwarz_inc = 1e-12;
     o->rebuild_every = 0;
     o->schwarz_fp32 = 1e-6;
+    o->guess = PF_GUESS_CORE;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }

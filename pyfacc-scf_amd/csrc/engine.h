@@ -70,11 +70,11 @@ class Engine {
     // call; two-electron quartets screened at tau on Q Q max|Gamma|.
     // terms (optional, 5 x natoms x 3): nuclear repulsion, kinetic (D dT),
     // nuclear attraction (D dV), overlap (-W dS), two-electron.
+    void gradient(double *grad_out, double tau, double *terms = nullptr);
     // ERI blocks of shell quartets quartets[4q..4q+3] (shell indices) in
     // Libint's layout, concatenated into out (host); offsets[q] (host, nq + 1
     // entries, may be null) is where block q starts.
     void eri_quartets(int nq, const int *quartets, double *out, long long *offsets);
-    void gradient(double *grad_out, double tau, double *terms = nullptr);
 
     // Fock-build statistics accumulated over an SCF, index precision*6 + class
     // (precision 0 = FP64, 1 = FP32): quartets, milliseconds of the main ERI
@@ -104,8 +104,13 @@ class Engine {
     unsigned long long collect_profile();
     void density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
                            double *eps_dev);
+    // Superposition of atomic densities (spin-summed, host, nbf^2 row-major).
+    std::vector<double> sad_density();
 
     Molecule mol_;
+    std::vector<std::string> species_;
+    std::string basis_path_;
+    bool renorm_ = true;
     Basis basis_;
     double schwarz_;
     hipStream_t st_ = nullptr;

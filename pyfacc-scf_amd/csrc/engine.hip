@@ -10,7 +10,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 #include <thread>
 
 #include "boys.h"
@@ -56,7 +59,7 @@ PairView<float> PairSet::view32() const {
 
 Engine::Engine(const Molecule &mol, const std::vector<std::string> &species, const std::string &basis_path,
                bool renormalise, double schwarz)
-    : mol_(mol), schwarz_(schwarz) {
+    : mol_(mol), species_(species), basis_path_(basis_path), renorm_(renormalise), schwarz_(schwarz) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error("no GPU device visible");
     basis_ = build_basis(basis_path, species, mol_, renormalise);
@@ -395,6 +398,82 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
     gemm_f64(n, false, true, occ, w1_.p, X_.p, 0.0, D, st_);            // occ X P X^T
 }
 
+// Spin-summed density of the isolated atom of element `el` (atomic number Z)
+// in the basis at `path`: UHF with n_alpha = ceil(Z/2), n_beta = floor(Z/2)
+// (Jacobi densities: robust for one-function atoms), then spherically
+// averaged shell block by shell block -- s-p blocks vanish, a p-p block keeps
+// a third of its trace on the diagonal -- so the guess has the atom's
+// symmetry whatever orbitals the open shell settled in.  Cached per (basis
+// file, normalisation, element); the atom's basis functions in shell order.
+static std::vector<double> atomic_density(const std::string &path, bool renorm, const std::string &el, int Z) {
+    static std::mutex mu;
+    static std::map<std::tuple<std::string, bool, std::string>, std::vector<double>> cache;
+    const auto key = std::make_tuple(path, renorm, el);
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    Molecule atom;
+    atom.natoms = 1;
+    atom.Z = {Z};
+    atom.xyz = {0.0, 0.0, 0.0};
+    Engine eng(atom, {el}, path, renorm, 1e-14);
+    pf_options o;
+    pf_default_options(&o);
+    o.eps = 1e-9;
+    o.max_iter = 200;
+    o.diis = 8;
+    o.diag = PF_DIAG_JACOBI;
+    const int na = (Z + 1) / 2, nb = Z / 2, n = eng.nbf();
+    const size_t n2 = (size_t)n * n;
+    std::vector<double> Ds(2 * n2, 0.0), D(n2, 0.0);
+    pf_result r;
+    eng.scf(na, nb, o, r, Ds.data(), nullptr);
+    const bool uhf = na != nb;
+    for (size_t e = 0; e < n2; ++e) D[e] = uhf ? Ds[e] + Ds[n2 + e] : Ds[e];
+    const auto &sh = eng.basis().shells;
+    for (size_t i = 0; i < sh.size(); ++i)
+        for (size_t j = 0; j < sh.size(); ++j) {
+            const int ni = ncart(sh[i].l), nj = ncart(sh[j].l);
+            if (sh[i].l == 0 && sh[j].l == 0) continue;
+            double tr = 0.0;
+            if (sh[i].l == sh[j].l)
+                for (int k = 0; k < ni; ++k) tr += D[(size_t)(sh[i].bf + k) * n + sh[j].bf + k];
+            for (int a = 0; a < ni; ++a)
+                for (int b = 0; b < nj; ++b)
+                    D[(size_t)(sh[i].bf + a) * n + sh[j].bf + b] =
+                        (sh[i].l == sh[j].l && a == b) ? tr / ni : 0.0;
+        }
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = D;
+    return D;
+}
+
+std::vector<double> Engine::sad_density() {
+    const int n = nbf();
+    std::vector<double> D((size_t)n * n, 0.0);
+    const auto &sh = basis_.shells;
+    for (int a = 0; a < mol_.natoms; ++a) {
+        const std::vector<double> Da = atomic_density(basis_path_, renorm_, species_[a], mol_.Z[a]);
+        // the atom's shells in the molecule, in the order they have on the isolated atom
+        std::vector<int> mine;
+        for (int s = 0; s < (int)sh.size(); ++s)
+            if (sh[s].atom == a) mine.push_back(s);
+        int na = 0;
+        std::vector<int> off;
+        for (int s : mine) { off.push_back(na); na += ncart(sh[s].l); }
+        if ((size_t)na * na != Da.size()) throw Error("SAD guess: atomic basis mismatch for " + species_[a]);
+        for (size_t i = 0; i < mine.size(); ++i)
+            for (size_t j = 0; j < mine.size(); ++j)
+                for (int p = 0; p < ncart(sh[mine[i]].l); ++p)
+                    for (int q = 0; q < ncart(sh[mine[j]].l); ++q)
+                        D[(size_t)(sh[mine[i]].bf + p) * n + sh[mine[j]].bf + q] =
+                            Da[(size_t)(off[i] + p) * na + off[j] + q];
+    }
+    return D;
+}
+
 void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out) {
     const double t0 = now();
     res = pf_result{};
@@ -438,8 +517,17 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     const double occ = uhf ? 1.0 : 2.0;
     DevBuf<double> D(n2 * nd), Dold(n2 * nd), Dbuilt(n2 * nd), dD(n2 * nd), G(n2 * nd), F(n2 * nd), red(4);
     DevBuf<double> part((size_t)RED_BLOCKS * DotPtrs::MAX);  // reduction partials
-    // core guess (rscf.f90:113-114; uscf.f90:140-144, C_alpha = C_beta)
-    {
+    if (opt.guess == PF_GUESS_SAD) {
+        // superposition of atomic densities: RHF D = the sum (occupation 2 folded
+        // in), UHF D_alpha = D_beta = half of it
+        const std::vector<double> sad = sad_density();
+        D.upload(sad.data(), n2, st_);
+        if (uhf) {
+            dev_axpby(n2, 0.5, D.p, 0.0, D.p, D.p + n2, st_);
+            dev_axpby(n2, 0.5, D.p, 0.0, D.p, D.p, st_);
+        }
+    } else {
+        // core guess (rscf.f90:113-114; uscf.f90:140-144, C_alpha = C_beta)
         const double td = now();
         for (int s = 0; s < nd; ++s) density_from_fock(H_.p, nocc[s], occ, D.p + s * n2, opt, nullptr);
         PF_HIP(hipStreamSynchronize(st_));

@@ -97,3 +97,19 @@ def test_water_dimer_modes_agree(gpu_lib, precision, incremental):
     r = pyfacc.scf(sp, xyz, 10, 10, eps=1e-9, precision=precision, incremental=incremental)
     assert base["converged"] and r["converged"]
     assert abs(r["energy"] - base["energy"]) < 1e-8
+
+
+@pytest.mark.parametrize("name,na,nb,prec", [("h2o_16.xyz", 80, 80, 0), ("h2o_16.xyz", 80, 80, 1),
+                                             ("h2o_15oh.xyz", 80, 79, 0), ("h2o_c1.xyz", 5, 4, 0)])
+def test_sad_guess_same_energy(gpu_lib, name, na, nb, prec):
+    """The superposition-of-atomic-densities start converges to the core
+    guess's energy (the guess changes the path, not the fixed point)."""
+    import pyfacc
+    sp, xyz = load_xyz(name)
+    kw = dict(eps=1e-9, max_iter=150, diis=8, schwarz=1e-11, precision=prec)
+    core = pyfacc.scf(sp, xyz, na, nb, guess=0, **kw)
+    sad = pyfacc.scf(sp, xyz, na, nb, guess=1, **kw)
+    assert core["converged"] and sad["converged"]
+    assert abs(core["energy"] - sad["energy"]) < 1e-8
+    if name == "h2o_16.xyz":
+        assert sad["iterations"] < core["iterations"]
