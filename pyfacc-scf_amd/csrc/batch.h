@@ -100,6 +100,7 @@ class BatchEngine {
     std::vector<long long> moff_, poff_;
     long long mtot_ = 0, ptot_ = 0;
     std::vector<Shell> shells_;
+    std::vector<int> shoff_;  // molecule m's shells: shells_[shoff_[m] .. shoff_[m+1])
     DevBuf<Shell> d_shells_;
     // device
     DevBuf<double> tab_;

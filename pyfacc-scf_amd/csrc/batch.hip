@@ -476,6 +476,7 @@ void BatchEngine::setup() {
         nmax_ = std::max(nmax_, n);
         for (int s = 0; s <= uhf; ++s) { pn_.push_back(n); pmol_.push_back(m); }
     }
+    shoff_ = shoff;
     P_ = (int)pn_.size();
     moff_.resize(M_);
     poff_.resize(P_);
@@ -821,9 +822,24 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mnstore_.p);
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_muse_.p);
     for (auto *b : {&d_E_, &d_Eold_, &d_iters_, &d_conv_}) dev_fill(M_, 0.0, b->p, st_);
-    // core guess: D from H (rscf.f90:113-114; uscf.f90:140-144)
-    bcopy_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, H_.p, d_pmoff_.p, Fx_.p);
-    density_step(Fx_.p);
+    if (opt.guess == PF_GUESS_SAD) {
+        // superposition of atomic densities (as Engine::scf): RHF the sum, UHF half per spin
+        std::vector<double> hD((size_t)ptot_, 0.0);
+        for (int m = 0; m < M_; ++m) {
+            const int n = mn_[m], p0 = mprob_[m];
+            const std::vector<double> Dm = sad_density(shells_.data() + shoff_[m], shoff_[m + 1] - shoff_[m], n,
+                                                       mols_[m].species, mols_[m].mol.Z, basis_path_, renorm_);
+            const double f = muhf_[m] ? 0.5 : 1.0;
+            for (int s = 0; s <= muhf_[m]; ++s)
+                for (size_t e = 0; e < (size_t)n * n; ++e) hD[poff_[p0 + s] + e] = f * Dm[e];
+        }
+        D_.upload(hD.data(), hD.size(), st_);
+        bdtot_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, D_.p, Dtot_.p);
+    } else {
+        // core guess: D from H (rscf.f90:113-114; uscf.f90:140-144)
+        bcopy_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, H_.p, d_pmoff_.p, Fx_.p);
+        density_step(Fx_.p);
+    }
     // autotune the digest's wavefronts per molecule (cached per device and batch shape)
     hipDeviceProp_t prop;
     int dev = 0;

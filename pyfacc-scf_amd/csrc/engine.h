@@ -47,6 +47,14 @@ struct Timing {
     int calls = 0;
 };
 
+// Spin-summed density of the isolated atom (element label, atomic number Z)
+// in the basis at path, spherically averaged per shell block; cached.
+std::vector<double> atomic_density(const std::string &path, bool renorm, const std::string &el, int Z);
+// Superposition of atomic densities over shells sh[0..nsh) (atom indices and
+// first basis functions local to the molecule): nbf^2, row-major, spin-summed.
+std::vector<double> sad_density(const Shell *sh, int nsh, int nbf, const std::vector<std::string> &species,
+                                const std::vector<int> &Z, const std::string &path, bool renorm);
+
 class Engine {
    public:
     Engine(const Molecule &mol, const std::vector<std::string> &species, const std::string &basis_path,

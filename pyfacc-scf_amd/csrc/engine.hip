@@ -405,7 +405,7 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
 // a third of its trace on the diagonal -- so the guess has the atom's
 // symmetry whatever orbitals the open shell settled in.  Cached per (basis
 // file, normalisation, element); the atom's basis functions in shell order.
-static std::vector<double> atomic_density(const std::string &path, bool renorm, const std::string &el, int Z) {
+std::vector<double> atomic_density(const std::string &path, bool renorm, const std::string &el, int Z) {
     static std::mutex mu;
     static std::map<std::tuple<std::string, bool, std::string>, std::vector<double>> cache;
     const auto key = std::make_tuple(path, renorm, el);
@@ -450,20 +450,20 @@ static std::vector<double> atomic_density(const std::string &path, bool renorm, 
     return D;
 }
 
-std::vector<double> Engine::sad_density() {
-    const int n = nbf();
+std::vector<double> sad_density(const Shell *sh, int nsh, int nbf, const std::vector<std::string> &species,
+                                const std::vector<int> &Z, const std::string &path, bool renorm) {
+    const int n = nbf;
     std::vector<double> D((size_t)n * n, 0.0);
-    const auto &sh = basis_.shells;
-    for (int a = 0; a < mol_.natoms; ++a) {
-        const std::vector<double> Da = atomic_density(basis_path_, renorm_, species_[a], mol_.Z[a]);
+    for (int a = 0; a < (int)species.size(); ++a) {
+        const std::vector<double> Da = atomic_density(path, renorm, species[a], Z[a]);
         // the atom's shells in the molecule, in the order they have on the isolated atom
         std::vector<int> mine;
-        for (int s = 0; s < (int)sh.size(); ++s)
+        for (int s = 0; s < nsh; ++s)
             if (sh[s].atom == a) mine.push_back(s);
         int na = 0;
         std::vector<int> off;
         for (int s : mine) { off.push_back(na); na += ncart(sh[s].l); }
-        if ((size_t)na * na != Da.size()) throw Error("SAD guess: atomic basis mismatch for " + species_[a]);
+        if ((size_t)na * na != Da.size()) throw Error("SAD guess: atomic basis mismatch for " + species[a]);
         for (size_t i = 0; i < mine.size(); ++i)
             for (size_t j = 0; j < mine.size(); ++j)
                 for (int p = 0; p < ncart(sh[mine[i]].l); ++p)
@@ -472,6 +472,10 @@ std::vector<double> Engine::sad_density() {
                             Da[(size_t)(off[i] + p) * na + off[j] + q];
     }
     return D;
+}
+
+std::vector<double> Engine::sad_density() {
+    return pf::sad_density(basis_.shells.data(), nshell(), nbf(), species_, mol_.Z, basis_path_, renorm_);
 }
 
 void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out) {

@@ -94,3 +94,17 @@ def test_batch_plain_roothaan_and_errors(gpu_lib):
     sp, xyz, na, nb = mols[0]
     with pytest.raises(pyfacc.PyfaccError):
         pyfacc.batch_scf([(sp, xyz, 40, 40)])
+
+
+def test_batch_sad_guess_same_energies(gpu_lib, batch64):
+    """The superposition-of-atomic-densities start reaches the same RHF
+    energies in fewer iterations.  UHF (the OH clusters) has several
+    stationary points and a different start may settle on another one, so
+    only their convergence is required."""
+    import pyfacc
+    mols, r = batch64
+    s = pyfacc.batch_scf(mols, graph=True, guess=1, **OPTS)
+    assert s["converged"].all()
+    closed = np.array([na == nb for _, _, na, nb in mols])
+    assert np.abs(s["energy"] - r["energy"])[closed].max() < 1e-8
+    assert s["iterations"][closed].sum() < r["iterations"][closed].sum()
