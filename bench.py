@@ -122,6 +122,7 @@ def batched(pyfacc, count):
         res[name] = (time.perf_counter() - t0, r)
     (t64, r64), (tmx, rmx) = res["fp64"], res["mixed"]
     both = r64["converged"] & rmx["converged"]
+    closed = np.array([m[2] == m[3] for m in mols])
     st64, stmx = r64["stats"], rmx["stats"]
     return {
         "instances": count,
@@ -143,9 +144,11 @@ def batched(pyfacc, count):
         "converged_mixed": int(rmx["converged"].sum()),
         # over the molecules converged in both forms (the others stop at max_iter)
         "max_abs_dE_mixed_vs_fp64": float(np.abs(rmx["energy"] - r64["energy"])[both].max()),
-        # instances whose two precisions land more than 1e-6 Eh apart (a different
-        # UHF stationary point, not rounding: the single-molecule engine agrees)
+        # instances whose two precisions land more than 1e-6 Eh apart: UHF ones
+        # settling on different stationary points (not rounding), split by kind
         "n_abs_dE_above_1e-6": int((np.abs(rmx["energy"] - r64["energy"])[both] > 1e-6).sum()),
+        "max_abs_dE_mixed_vs_fp64_rhf": float(np.abs(rmx["energy"] - r64["energy"])[both & closed].max()),
+        "n_abs_dE_above_1e-6_uhf": int((np.abs(rmx["energy"] - r64["energy"])[both & ~closed] > 1e-6).sum()),
         "guess": "SAD",
         "hipgraph": bool(st64["graph"]),
         "digest_waves_autotuned": int(st64["digest_waves"]),
