@@ -275,7 +275,7 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
     PendingBuild pb{pbase, {}};
     hipEvent_t *ev = pb.ev;
     if (profile_classes)
-        for (int e = 0; e < 12; ++e) PF_HIP(hipEventCreate(&ev[e]));
+        for (int e = 0; e < 12; ++e) ev[e] = event_acquire();
     for (size_t q = 0; q < classes_.size(); ++q) {
         const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
         if constexpr (std::is_same<R, double>::value) { a.bra = B.view64(); a.ket = K.view64(); }
@@ -318,7 +318,7 @@ unsigned long long Engine::collect_profile() {
             PF_HIP(hipEventElapsedTime(&ms, b.ev[2 * q], b.ev[2 * q + 1]));
             prof_ms[b.pbase + classes_[q].slot] += ms;
         }
-        for (auto &e : b.ev) (void)hipEventDestroy(e);
+        for (auto &e : b.ev) event_release(e);
     }
     pending_.clear();
     return total;
@@ -564,7 +564,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     auto mark = [&](int k) {
         if (k == 0) {
             tev.emplace_back();
-            for (auto &e : tev.back()) PF_HIP(hipEventCreate(&e));
+            for (auto &e : tev.back()) e = event_acquire();
         }
         PF_HIP(hipEventRecord(tev.back()[k], st_));
     };
@@ -729,7 +729,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         PF_HIP(hipEventElapsedTime(&b, e3[1], e3[2]));
         res.t_fock += 1e-3 * a;
         res.t_diag += 1e-3 * b;
-        for (auto &e : e3) (void)hipEventDestroy(e);
+        for (auto &e : e3) event_release(e);
     }
     res.iterations = std::min(it, opt.max_iter);
     res.converged = conv ? 1 : 0;

@@ -103,6 +103,10 @@ void pool_release(PooledBlock &b);
 // same thread, ordered after its last use.  dev_free takes the requested size.
 void *dev_alloc(size_t bytes);
 void dev_free(void *p, size_t bytes);
+// HIP events from a per-thread pool: release only events whose recorded work
+// has completed (after a synchronisation or a successful elapsed-time read).
+hipEvent_t event_acquire();
+void event_release(hipEvent_t e);
 
 // Device buffer with RAII.
 template <typename T>

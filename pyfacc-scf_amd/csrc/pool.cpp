@@ -148,4 +148,34 @@ void dev_free(void *p, size_t bytes) {
     (void)hipFree(p);
 }
 
+// ---- per-thread pool of HIP events
+//
+// An SCF records a few hundred timing events; creating and destroying them one
+// by one cost milliseconds of host time per call.  Events go back to the
+// calling thread's pool once read (the work they bracket is complete) and are
+// recorded again by later calls; like the block cache, the pool is never torn
+// down.
+namespace {
+std::vector<hipEvent_t> &event_pool() {
+    static thread_local std::vector<hipEvent_t> *v = new std::vector<hipEvent_t>();
+    return *v;
+}
+}  // namespace
+
+hipEvent_t event_acquire() {
+    auto &pool = event_pool();
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    PF_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void event_release(hipEvent_t e) {
+    if (e) event_pool().push_back(e);
+}
+
 }  // namespace pf
