@@ -102,6 +102,31 @@ def cpu_baseline(species, xyz, n_builds, target_s):
     }
 
 
+def reference_inputs(pyfacc, common):
+    """Mixed and FP64 SCFs of two of the reference's own inputs (a 60-residue
+    glycine chain, 1387 bf; an RNA fragment with P, 491 bf), one untimed run
+    each first.  The RNA fragment uses the orthonormal DIIS error."""
+    res = {}
+    for name, extra in (("gly60.xyz", {}), ("rna1.xyz", {"diis_basis": 1})):
+        sp, xyz = load_xyz(os.path.join(ROOT, "pyfacc-scf_amd", "data", "molecules", name))
+        nocc = sum(pyfacc.helper_functions.periodic_table[s] for s in sp) // 2
+        row = {"natoms": len(sp)}
+        for form, prec in (("mixed", 1), ("fp64", 0)):
+            o = pyfacc.default_options(precision=prec, **dict(common, max_iter=150, **extra))
+            pyfacc.scf(sp, xyz, nocc, nocc, options=o)
+            t0 = time.perf_counter()
+            r = pyfacc.scf(sp, xyz, nocc, nocc, options=o)
+            row[f"scf_time_{form}_s"] = time.perf_counter() - t0
+            row[f"iterations_{form}"] = r["iterations"]
+            row[f"converged_{form}"] = bool(r["converged"])
+            row[f"energy_{form}"] = r["energy"]
+            row["nbf"] = r["nbf"]
+        row["dE_mixed_vs_fp64"] = row["energy_mixed"] - row["energy_fp64"]
+        row["speedup_mixed_vs_fp64"] = row["scf_time_fp64_s"] / row["scf_time_mixed_s"]
+        res[name[:-4]] = row
+    return res
+
+
 def batched(pyfacc, count):
     """The batched workload: `count` seeded clusters of 1-12 waters (about 10 %
     with an OH radical, UHF), STO-3G, each to |dE| < 1e-8 and ||dD|| < 1e-8
@@ -165,6 +190,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batch", type=int, default=4096,
                     help="instances of the batched water-cluster workload (0: skip)")
+    ap.add_argument("--no-reference-inputs", action="store_true",
+                    help="skip the SCFs of the reference's glycine-chain and RNA inputs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the byte-counter "
                          "calibration kernel for rocprofv3 --pmc runs)")
@@ -300,6 +327,8 @@ def main():
                                          "tflops": fl64 / (avg64 * 1e-3) / 1e12},
         },
     }
+    if not args.no_reference_inputs:
+        out["reference_inputs"] = reference_inputs(pyfacc, common)
     if args.batch > 0:
         out["batched"] = batched(pyfacc, args.batch)
     if not args.no_cpu_baseline:

@@ -53,6 +53,7 @@ enum { PF_PREC_FP64 = 0, PF_PREC_MIXED = 1 };
 enum { PF_ERI_DIRECT = 0, PF_ERI_STORED = 1 };
 enum { PF_DIAG_SP2 = 0, PF_DIAG_JACOBI = 1 };
 enum { PF_GUESS_CORE = 0, PF_GUESS_SAD = 1 };
+enum { PF_DIIS_AO = 0, PF_DIIS_ORTHO = 1 };
 
 typedef struct pf_options {
     double eps;           /* convergence threshold on |dE| and ||dD||_F (rscf.f90:152-155) */
@@ -73,6 +74,12 @@ typedef struct pf_options {
                              FP32 incremental builds before the full FP64 rebuild that replaces their sum */
     int guess;            /* PF_GUESS_CORE: core Hamiltonian (rscf.f90:113-114, the reference's); PF_GUESS_SAD:
                              superposition of spherically averaged atomic UHF densities */
+    double level_shift;   /* Eh added to the virtual space, F + b (S - S D S / occ), after DIIS extrapolation,
+                             while ||dD||_F of the previous iteration is >= level_shift_until (0: off;
+                             pf_scf and pf_scf_gradient, not pf_batch_scf) */
+    double level_shift_until;
+    int diis_basis;       /* DIIS error vector: PF_DIIS_AO, F D S - S D F; PF_DIIS_ORTHO, X^T (F D S - S D F) X
+                             with X the orthogonaliser (pf_scf and pf_scf_gradient) */
 } pf_options;
 
 typedef struct pf_result {
@@ -97,7 +104,7 @@ typedef struct pf_result {
 
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
- * rebuild_every 0, schwarz_fp32 1e-6, core guess. */
+ * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

@@ -105,6 +105,9 @@ void pf_default_options(pf_options *o) {
     o->rebuild_every = 0;
     o->schwarz_fp32 = 1e-6;
     o->guess = PF_GUESS_CORE;
+    o->level_shift = 0.0;
+    o->level_shift_until = 0.1;
+    o->diis_basis = PF_DIIS_AO;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }
@@ -188,6 +191,8 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
         if (opt) o = *opt;
         else pf_default_options(&o);
         if (o.max_iter < 1) throw Error("max_iter must be >= 1");
+        if (!(o.level_shift >= 0.0)) throw Error("level_shift must be >= 0");
+        if (o.diis_basis != PF_DIIS_AO && o.diis_basis != PF_DIIS_ORTHO) throw Error("diis_basis must be 0 or 1");
         Input in = make_input(natoms, species, z, coords);
         Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,
                    o.schwarz > 0 ? o.schwarz : 1e-14);
@@ -245,6 +250,8 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
         if (opt) o = *opt;
         else pf_default_options(&o);
         if (o.max_iter < 1) throw Error("max_iter must be >= 1");
+        if (!(o.level_shift >= 0.0)) throw Error("level_shift must be >= 0");
+        if (o.diis_basis != PF_DIIS_AO && o.diis_basis != PF_DIIS_ORTHO) throw Error("diis_basis must be 0 or 1");
         if (!grad) throw Error("null gradient output");
         Input in = make_input(natoms, species, z, coords);
         Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,

@@ -547,6 +547,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     DevBuf<double> dots(std::max(ndiis, 1)), Fx(ndiis ? n2 * nd : 1), Bdev(std::max(ndiis * ndiis, 1)),
         coef(std::max(ndiis, 1));
     DevBuf<int> dok(1);
+    DevBuf<double> Fls(opt.level_shift > 0.0 ? n2 * nd : 1);  // level-shifted Fock
     DevBuf<double> dots_all(opt.precision == PF_PREC_MIXED && ndiis ? (size_t)ndiis * ndiis : 1);
     int nstore = 0, head = 0;
 
@@ -561,6 +562,15 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     // per iteration: events at the start of the Fock build, its end, and the end
     // of the density/energy step (t_fock and t_diag without host round trips)
     std::vector<std::array<hipEvent_t, 3>> tev;
+    // DIIS error in the orthonormal basis on request, e' = X^T e X (in place; w1_
+    // as scratch): the AO commutator weights each element by the overlap metric;
+    // on large molecules with small gaps the orthonormal form can take fewer
+    // steps (either may end on another stationary point of UHF)
+    auto to_orth = [&](double *e) {
+        if (opt.diis_basis != PF_DIIS_ORTHO) return;
+        gemm_f64(n, true, false, 1.0, X_.p, e, 0.0, w1_.p, st_);
+        gemm_f64(n, false, false, 1.0, w1_.p, X_.p, 0.0, e, st_);
+    };
     auto mark = [&](int k) {
         if (k == 0) {
             tev.emplace_back();
@@ -627,7 +637,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                 // The stored DIIS Fock matrices share (nearly) this offset
                 // d = G_fp32 - G_fp64, accumulated mostly by the first builds:
                 // remove it from each F_k
-                // and its error vector, e_k -= d D S - S D d (with the current D for
+                // and its error vector, e_k -= d D S - S D d (or X^T . X; with the current D for
                 // D_k, an error of |d| |D_k - D|), and rebuild the B matrix.  The
                 // subspace then continues instead of restarting.
                 dev_axpby(n2 * nd, 1.0, Gs.p, -1.0, G.p, Gs.p, st_);
@@ -635,6 +645,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                     gemm_f64(n, false, false, 1.0, Gs.p + s * n2, D.p + s * n2, 0.0, w1_.p, st_);
                     gemm_f64(n, false, false, 1.0, w1_.p, S_.p, 0.0, w2_.p, st_);
                     dev_antisym(n, w2_.p, dD.p + s * n2, st_);
+                    to_orth(dD.p + s * n2);
                 }
                 for (int k = 0; k < nstore; ++k) {
                     dev_axpby(n2 * nd, 1.0, dF[k].p, -1.0, Gs.p, dF[k].p, st_);
@@ -670,12 +681,13 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         PF_HIP(hipMemcpyAsync(Dold.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
         const double *Fuse = F.p;
         if (ndiis) {
-            // error e = F D S - S D F per spin, with the density F was built from
+            // error e = F D S - S D F per spin (or X^T e X), with the density F was built from
             const int slot = head;
             for (int s = 0; s < nd; ++s) {
                 gemm_f64(n, false, false, 1.0, F.p + s * n2, D.p + s * n2, 0.0, w1_.p, st_);
                 gemm_f64(n, false, false, 1.0, w1_.p, S_.p, 0.0, w2_.p, st_);
                 dev_antisym(n, w2_.p, dE[slot].p + s * n2, st_);
+                to_orth(dE[slot].p + s * n2);
             }
             PF_HIP(hipMemcpyAsync(dF[slot].p, F.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
             head = (head + 1) % ndiis;
@@ -696,6 +708,20 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
             } else {
                 dev_diis_solve(ndiis, nstore, slot, dots.p, Bdev.p, coef.p, dok.p, st_);  // records the B entries
             }
+        }
+        if (opt.level_shift > 0.0 && last_dD >= opt.level_shift_until) {
+            // virtual-space level shift: F + b (S - S D S / occ), with the density
+            // F was built from (S D S / occ is the occupied projector in the AO
+            // metric); the converged solution, where D is that projector of F,
+            // does not depend on b
+            const double b = opt.level_shift;
+            for (int s = 0; s < nd; ++s) {
+                gemm_f64(n, false, false, 1.0, S_.p, D.p + s * n2, 0.0, w1_.p, st_);
+                gemm_f64(n, false, false, 1.0, w1_.p, S_.p, 0.0, w2_.p, st_);
+                dev_axpby(n2, 1.0, Fuse + s * n2, b, S_.p, Fls.p + s * n2, st_);
+                dev_axpby(n2, 1.0, Fls.p + s * n2, -b / occ, w2_.p, Fls.p + s * n2, st_);
+            }
+            Fuse = Fls.p;
         }
         for (int s = 0; s < nd; ++s) density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr);
         // energy with the new D and the F built this iteration (rscf.f90:144-151; uscf.f90:186-193)

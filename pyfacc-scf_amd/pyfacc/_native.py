@@ -21,7 +21,9 @@ class Options(ctypes.Structure):
                 ("incremental", ctypes.c_int), ("eri_mode", ctypes.c_int), ("diis", ctypes.c_int),
                 ("diag", ctypes.c_int), ("renormalize", ctypes.c_int), ("schwarz", ctypes.c_double),
                 ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int), ("schwarz_inc", ctypes.c_double),
-                ("rebuild_every", ctypes.c_int), ("schwarz_fp32", ctypes.c_double), ("guess", ctypes.c_int)]
+                ("rebuild_every", ctypes.c_int), ("schwarz_fp32", ctypes.c_double), ("guess", ctypes.c_int),
+                ("level_shift", ctypes.c_double), ("level_shift_until", ctypes.c_double),
+                ("diis_basis", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
