@@ -65,8 +65,7 @@ typedef struct pf_options {
     int diag;             /* PF_DIAG_SP2 (GEMM spectral projector) or PF_DIAG_JACOBI */
     int renormalize;      /* 1: contracted functions normalised (Libint); 0: primitive norms only (reference) */
     double schwarz;       /* screening threshold on Q_ab Q_cd max|D| */
-    double mixed_switch;  /* mixed precision: ||dD||_F below which the FP64 finish starts; one full
-                             FP64 rebuild inside the FP32 phase at 1e4 * mixed_switch (pf_scf) */
+    double mixed_switch;  /* mixed precision: ||dD||_F below which the FP64 finish starts */
     int verbose;          /* 1: one line per iteration on stderr */
     double schwarz_inc;   /* screening threshold of FP64 incremental builds (on Q Q max|dD|) */
     int rebuild_every;    /* incremental FP64 builds: full rebuild after this many increments (0: never) */
@@ -80,6 +79,8 @@ typedef struct pf_options {
     double level_shift_until;
     int diis_basis;       /* DIIS error vector: PF_DIIS_AO, F D S - S D F; PF_DIIS_ORTHO, X^T (F D S - S D F) X
                              with X the orthogonaliser (pf_scf and pf_scf_gradient) */
+    double mixed_anchor;  /* mixed precision: the full FP64 rebuild inside the FP32 phase comes when ||dD||_F
+                             first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient) */
 } pf_options;
 
 typedef struct pf_result {
@@ -104,7 +105,8 @@ typedef struct pf_result {
 
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
- * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error. */
+ * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error,
+ * mixed_anchor 0.1. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

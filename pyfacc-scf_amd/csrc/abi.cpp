@@ -108,6 +108,7 @@ void pf_default_options(pf_options *o) {
     o->level_shift = 0.0;
     o->level_shift_until = 0.1;
     o->diis_basis = PF_DIIS_AO;
+    o->mixed_anchor = 0.1;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }

@@ -149,8 +149,6 @@ class Engine {
     std::array<PooledBlock, 6> jbuf_;  // per class: J'_cd of each evaluated quartet (pooled)
     std::array<long long, 6> jcap_{};
     static constexpr long long JBUF_MAX_VALUES = 1LL << 27;  // per class (1 GiB), beyond: device atomics
-    // mixed precision: full FP64 rebuild once ||dD|| < MIXED_ANCHOR * mixed_switch
-    static constexpr double MIXED_ANCHOR = 1e4;
     // stored integrals
     bool store_ready_ = false;
     std::array<DevBuf<double>, 6> store_;

@@ -586,11 +586,11 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         // Mixed precision: the FP32 rounding of an incremental build scales with
         // |dD|, so the first builds (dD as large as D itself) leave almost all
         // of the accumulated error.  One full FP64 rebuild when ||dD|| first
-        // falls below MIXED_ANCHOR * mixed_switch clears it; the switch to FP64
+        // falls below mixed_anchor clears it; the switch to FP64
         // at mixed_switch then moves the density by little more than a normal
         // step.
         const bool switch_now = fp32_phase && last_dD < opt.mixed_switch;
-        const bool anchor_now = fp32_phase && !switch_now && !anchored && last_dD < MIXED_ANCHOR * opt.mixed_switch;
+        const bool anchor_now = fp32_phase && !switch_now && !anchored && last_dD < opt.mixed_anchor;
         const bool rebuild = !fp32_phase && incremental && opt.rebuild_every > 0 &&
                              since_full >= opt.rebuild_every;
         if (switch_now && anchored) {

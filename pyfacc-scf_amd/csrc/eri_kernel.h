@@ -592,20 +592,37 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     JKArgs<R> a = args;
     a.chunk = jk_chunk<LA, LB, LC, LD>();
     const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK>;
-    int dev = 0, ncu = 0, maxlds = 0, per = 0;
+    // launch shape per (device, nbf), worked out once per thread: the
+    // attribute and occupancy queries cost host time on every build otherwise
+    struct Shape {
+        int dev = -1, nbf = -1, grid = 0;
+        bool krows = false;
+        size_t dyn = 0;
+    };
+    static thread_local Shape sh;
+    int dev = 0;
     PF_HIP(hipGetDevice(&dev));
-    PF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    PF_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    hipFuncAttributes fa;
-    PF_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern)));
-    const size_t rows = sizeof(double) * (size_t)NK * (NA + NB) * a.nbf, jab = sizeof(double) * NA * NB;
-    a.krows = fa.sharedSizeBytes + rows + jab <= (size_t)maxlds;
-    const size_t dyn = (a.krows ? rows : 0) + jab;
-    if (fa.sharedSizeBytes + dyn > 65536)
-        PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)dyn));
-    PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * W, dyn));
-    const int grid = std::max(1, per) * std::max(1, ncu);  // persistent: as many workgroups as fit at once
+    if (sh.dev != dev || sh.nbf != a.nbf) {
+        int ncu = 0, maxlds = 0, per = 0;
+        PF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        PF_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        hipFuncAttributes fa;
+        PF_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern)));
+        const size_t rows = sizeof(double) * (size_t)NK * (NA + NB) * a.nbf, jab = sizeof(double) * NA * NB;
+        const bool krows = fa.sharedSizeBytes + rows + jab <= (size_t)maxlds;
+        const size_t dyn = (krows ? rows : 0) + jab;
+        // the limit is process-wide state: always the device maximum, so every
+        // thread's cached shape stays launchable whatever another thread set
+        if (fa.sharedSizeBytes + dyn > 65536)
+            PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       maxlds - (int)fa.sharedSizeBytes));
+        PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * W, dyn));
+        sh = Shape{dev, a.nbf, std::max(1, per) * std::max(1, ncu), krows, dyn};  // persistent: all that fit
+    }
+    a.krows = sh.krows;
+    const size_t dyn = sh.dyn;
+    const int grid = sh.grid;
     jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
     jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);
     if (a.ev0) PF_HIP(hipEventRecord(a.ev0, st));

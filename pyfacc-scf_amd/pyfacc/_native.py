@@ -23,7 +23,7 @@ class Options(ctypes.Structure):
                 ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int), ("schwarz_inc", ctypes.c_double),
                 ("rebuild_every", ctypes.c_int), ("schwarz_fp32", ctypes.c_double), ("guess", ctypes.c_int),
                 ("level_shift", ctypes.c_double), ("level_shift_until", ctypes.c_double),
-                ("diis_basis", ctypes.c_int)]
+                ("diis_basis", ctypes.c_int), ("mixed_anchor", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
