@@ -1,0 +1,105 @@
+"""The HIP path against the committed vectors in tests/golden/oracle_fixtures.json
+(Boys from mpmath; integrals, J/K and energies from the oracle), through the
+C-ABI.  Bars: Boys 1e-13 relative; S/T/V and ERIs 1e-12; J/K 1e-11
+(checksums of the water-cluster matrices 1e-11 relative); energies 1e-9 Eh."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_xyz
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(GOLDEN, "oracle_fixtures.json")
+
+
+@pytest.fixture(scope="module")
+def fx():
+    with open(FIX) as fh:
+        return json.load(fh)
+
+
+def _geom(e):
+    return e["species"], np.array(e["xyz_bohr"]).reshape(-1, 3)
+
+
+def test_boys_vs_fixture(gpu_lib, fx):
+    import pyfacc
+    b = fx["boys"]
+    F = pyfacc._native.boys(b["nmax"], np.array(b["x"]))
+    ref = np.array(b["F"])
+    assert np.abs(F - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.all(np.abs(F - ref) <= 1e-13 * np.abs(ref) + 1e-300)
+
+
+@pytest.mark.parametrize("name", ["h2o", "oh"])
+@pytest.mark.parametrize("renorm", [True, False])
+def test_one_electron_vs_fixture(gpu_lib, fx, name, renorm):
+    import pyfacc
+    e = fx[name]
+    n = e["nbf"]
+    S, T, V = pyfacc.one_electron(*_geom(e), renormalize=renorm)
+    suffix = "" if renorm else "_prim_norm"
+    for M, key in ((S, "S"), (T, "T"), (V, "V")):
+        assert np.abs(M - np.reshape(e[key + suffix], (n, n))).max() < 1e-12, key + suffix
+
+
+def test_h2o_eris_vs_fixture(gpu_lib, fx):
+    import pyfacc
+    e = fx["h2o"]
+    packed = pyfacc._native.eri_packed(*_geom(e))
+    assert np.abs(packed - np.array(e["eri_packed"])).max() < 1e-12
+
+
+@pytest.mark.parametrize("stored", [False, True])
+def test_h2o_jk_vs_fixture(gpu_lib, fx, stored):
+    import pyfacc
+    e = fx["h2o"]
+    n = e["nbf"]
+    J, K = pyfacc.jk(*_geom(e), np.reshape(e["D"], (n, n)), stored=stored, schwarz=1e-16)
+    assert np.abs(J - np.reshape(e["J"], (n, n))).max() < 1e-11
+    assert np.abs(K - np.reshape(e["K"], (n, n))).max() < 1e-11
+
+
+@pytest.mark.parametrize("name", ["h2o", "oh"])
+def test_small_energies_vs_fixture(gpu_lib, fx, name):
+    import pyfacc
+    e = fx[name]
+    r = pyfacc.scf(*_geom(e), e["nalpha"], e["nbeta"], eps=1e-10, max_iter=200, diis=8)
+    assert r["converged"]
+    assert abs(r["energy"] - e["energy"]) < 1e-9
+
+
+def _seeded_density(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    C = rng.standard_normal((n, max(1, n // 2))) / np.sqrt(n)
+    return C @ C.T
+
+
+def _checksums(M):
+    i = np.arange(len(M))
+    sign = np.where(((i[:, None] * 7 + i[None, :] * 3) % 5) < 2, 1.0, -1.0)
+    return {"sum": M.sum(), "fro": np.linalg.norm(M), "signed": (M * sign).sum()}
+
+
+@pytest.mark.parametrize("key", ["h2o16_sto3g", "h2o4_sto6g"])
+def test_water_cluster_jk_checksums_and_energy_vs_fixture(gpu_lib, fx, key):
+    import pyfacc
+    e = fx[key]
+    sp, xyz = load_xyz("h2o_16.xyz")
+    nw = e["waters"]
+    sp, xyz = sp[:3 * nw], xyz[:3 * nw]
+    D = _seeded_density(e["nbf"], e["density_seed"])
+    for k, v in _checksums(D).items():  # the same density as the generator's
+        assert abs(v - e["D"][k]) <= 1e-13 * abs(e["D"][k])
+    J, K = pyfacc.jk(sp, xyz, D, basis=e["basis"], schwarz=1e-16)
+    for M, name in ((J, "J"), (K, "K")):
+        got = _checksums(M)
+        for k in ("sum", "fro", "signed"):
+            ref = e[name][k]
+            assert abs(got[k] - ref) <= 1e-11 * max(abs(ref), e[name]["fro"]), (name, k)
+    r = pyfacc.scf(sp, xyz, 5 * nw, 5 * nw, basis=e["basis"], eps=1e-10, max_iter=200, diis=8, schwarz=1e-12)
+    assert r["converged"]
+    assert abs(r["energy"] - e["energy"]) < 1e-9
