@@ -30,6 +30,8 @@ import numpy as np  # noqa: E402
 
 ANG2BOHR = 1.8897259886
 PEAK_FP64_TFLOPS = 78.6
+# contracted integrals per shell quartet of each class (Cartesian s = 1, p = 3)
+CLASS_ERIS = {"(ss|ss)": 1, "(ps|ss)": 3, "(ps|ps)": 9, "(pp|ss)": 9, "(pp|ps)": 27, "(pp|pp)": 81}
 
 
 def load_xyz(path):
@@ -310,6 +312,15 @@ def main():
         "fock_build_ms_fp64": 1e3 * fp64[-1]["t_fock"] / fp64[-1]["fock_builds_fp64"],
         "fock_build_tflops_fp64": fock64_flops / (fp64[-1]["t_fock"] / fp64[-1]["fock_builds_fp64"]) / 1e12,
         "converged": bool(mixed[-1]["converged"] and fp64[-1]["converged"]),
+        # throughput of one FP64 full Fock build (J/K kernel time of its six classes)
+        "fp64_build_throughput": {
+            "shell_quartets_per_s": float(sum(v["quartets_per_build"] for v in per_class.values()) /
+                                          (sum(v["ms_per_build"] for v in per_class.values()) * 1e-3)),
+            "contracted_eris_per_s": float(sum(v["quartets_per_build"] * CLASS_ERIS[c] for c, v in per_class.items()) /
+                                           (sum(v["ms_per_build"] for v in per_class.values()) * 1e-3)),
+            "primitive_quartets_per_s": float(sum(v["prim_quartets_per_build"] for v in per_class.values()) /
+                                              (sum(v["ms_per_build"] for v in per_class.values()) * 1e-3)),
+        },
         "fp64_build_per_class": per_class,
         "roofline": {
             "bound": "mfma",
