@@ -129,6 +129,30 @@ def reference_inputs(pyfacc, common):
     return res
 
 
+def gradient_timing(pyfacc, common):
+    """SCF + analytic nuclear gradient of (H2O)32 STO-3G (224 bf, 288
+    coordinates): wall time of pf_scf_gradient beside pf_scf alone, FP64 and
+    with the mixed-precision forward pass (the gradient itself is FP64)."""
+    from pyfacc import grad
+    sp, xyz = load_xyz(os.path.join(ROOT, "pyfacc-scf_amd", "data", "molecules", "h2o_32.xyz"))
+    row = {"natoms": len(sp)}
+    for form, prec in (("fp64", 0), ("mixed", 1)):
+        o = pyfacc.default_options(precision=prec, **common)
+        grad.scf_gradient(sp, xyz, 160, 160, options=o)
+        t0 = time.perf_counter()
+        e, g, r = grad.scf_gradient(sp, xyz, 160, 160, options=o)
+        t_grad = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pyfacc.scf(sp, xyz, 160, 160, options=o)
+        t_scf = time.perf_counter() - t0
+        row[f"scf_plus_gradient_s_{form}"] = t_grad
+        row[f"scf_s_{form}"] = t_scf
+        row[f"gradient_only_s_{form}"] = t_grad - t_scf
+        row[f"max_abs_gradient_{form}"] = float(np.abs(g).max())
+        row[f"translational_sum_{form}"] = float(np.abs(g.sum(axis=0)).max())
+    return row
+
+
 def batched(pyfacc, count):
     """The batched workload: `count` seeded clusters of 1-12 waters (about 10 %
     with an OH radical, UHF), STO-3G, each to |dE| < 1e-8 and ||dD|| < 1e-8
@@ -348,6 +372,7 @@ def main():
         out["dE_mixed_vs_cpu_oracle"] = emx - e_cpu
     if not args.no_reference_inputs:
         out["reference_inputs"] = reference_inputs(pyfacc, common)
+        out["gradient_h2o32"] = gradient_timing(pyfacc, common)
     if args.batch > 0:
         out["batched"] = batched(pyfacc, args.batch)
     if not args.no_cpu_baseline:

@@ -89,6 +89,11 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
         else hi = mid;
     }
     const int nket = g.same ? min(lo, i + 1) : lo;
+    // blockIdx.y takes one slice of the ket list (slices of a multiple of 64
+    // covering all K.n kets), so a class with few bra pairs still fills the GPU
+    const int slice = ((K.n + (int)gridDim.y - 1) / (int)gridDim.y + 63) & ~63;
+    const int jbeg = (int)blockIdx.y * slice, jend = min(nket, jbeg + slice);
+    if (jbeg >= jend) return;
     const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
     const double ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
     const double deg_ab = (a == b) ? 0.5 : 1.0;
@@ -97,9 +102,9 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
     unsigned long long counted = 0;
 
-    for (int j0 = 0; j0 < nket; j0 += 64) {
+    for (int j0 = jbeg; j0 < jend; j0 += 64) {
         const int j = j0 + lane;
-        if (j >= nket) break;
+        if (j >= jend) break;
         const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
         {
             const double ab = g.Dsh[a * ns + b], cd = g.Dsh[c * ns + d];
@@ -238,8 +243,11 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
 template <int LA, int LB, int LC, int LD>
 void launch_grad2e(int nk, const GradArgs &g, hipStream_t st) {
     if (g.bra.n == 0) return;
-    if (nk == 1) grad2e_kernel<LA, LB, LC, LD, 1><<<g.bra.n, 64, 0, st>>>(g);
-    else grad2e_kernel<LA, LB, LC, LD, 2><<<g.bra.n, 64, 0, st>>>(g);
+    // one workgroup per (bra pair, ket slice): slices of at least 64 kets, at
+    // most 64 of them
+    const dim3 grid(g.bra.n, std::max(1, std::min(64, (g.ket.n + 63) / 64)));
+    if (nk == 1) grad2e_kernel<LA, LB, LC, LD, 1><<<grid, 64, 0, st>>>(g);
+    else grad2e_kernel<LA, LB, LC, LD, 2><<<grid, 64, 0, st>>>(g);
 }
 
 }  // namespace pf
