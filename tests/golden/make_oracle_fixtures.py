@@ -16,7 +16,9 @@ read (test infrastructure; nothing here ships).
 
 Every number is written with repr(), so the JSON holds it exactly.  Run from
 the repository root: python tests/golden/make_oracle_fixtures.py (a few
-minutes on 8 cores, most of it the (H2O)16 part)."""
+minutes on 8 cores, most of it the (H2O)16 part).  With --h2o16-sto6g it
+only adds the full (H2O)16 cluster in STO-6G (two passes over its 6.8e9
+primitive quartets: about half an hour on 8 cores) to the existing file."""
 import json
 import os
 import sys
@@ -86,7 +88,28 @@ def small(name, na, nb):
     return m, out
 
 
+def cluster_entry(nw, basis):
+    sp, xyz = load_xyz("h2o_16.xyz")
+    sp, xyz = sp[:3 * nw], xyz[:3 * nw]
+    m = oracle.Molecule(sp, xyz, basis)
+    r = m.scf(5 * nw, 5 * nw, eps=1e-10, max_iter=200, diis=8)
+    assert r["converged"]
+    D = seeded_density(m.nbf, 16)
+    J, K, _ = m.jk_direct(D)
+    return {"basis": basis, "waters": nw, "nbf": m.nbf, "energy": r["energy"], "iterations": r["iterations"],
+            "density_seed": 16, "D": checksums(D), "J": checksums(J), "K": checksums(K)}
+
+
 def main():
+    path = os.path.join(HERE, "oracle_fixtures.json")
+    if "--h2o16-sto6g" in sys.argv:
+        with open(path) as fh:
+            fx = json.load(fh)
+        fx["h2o16_sto6g"] = cluster_entry(16, "sto-6g")
+        print("h2o16_sto6g", fx["h2o16_sto6g"]["energy"], flush=True)
+        with open(path, "w") as fh:
+            json.dump(fx, fh)
+        return
     fx = {"boys": {"x": BOYS_X, "nmax": 5, "F": [[boys_mp(n, x) for n in range(6)] for x in BOYS_X]}}
     m, h2o = small("h2o_c1.xyz", 5, 5)
     h2o["eri_packed"] = mat(m.eri_packed())
@@ -97,17 +120,9 @@ def main():
     fx["h2o"] = h2o
     fx["oh"] = small("oh.xyz", 5, 4)[1]
     for key, nw, basis in (("h2o16_sto3g", 16, "sto-3g"), ("h2o4_sto6g", 4, "sto-6g")):
-        sp, xyz = load_xyz("h2o_16.xyz")
-        sp, xyz = sp[:3 * nw], xyz[:3 * nw]
-        m = oracle.Molecule(sp, xyz, basis)
-        r = m.scf(5 * nw, 5 * nw, eps=1e-10, max_iter=200, diis=8)
-        assert r["converged"]
-        D = seeded_density(m.nbf, 16)
-        J, K, _ = m.jk_direct(D)
-        fx[key] = {"basis": basis, "waters": nw, "nbf": m.nbf, "energy": r["energy"], "iterations": r["iterations"],
-                   "density_seed": 16, "D": checksums(D), "J": checksums(J), "K": checksums(K)}
-        print(key, r["energy"], flush=True)
-    with open(os.path.join(HERE, "oracle_fixtures.json"), "w") as fh:
+        fx[key] = cluster_entry(nw, basis)
+        print(key, fx[key]["energy"], flush=True)
+    with open(path, "w") as fh:
         json.dump(fx, fh)
 
 
