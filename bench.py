@@ -362,14 +362,6 @@ def main():
                                          "tflops": fl64 / (avg64 * 1e-3) / 1e12},
         },
     }
-    # the CPU oracle's energy of the same cluster (a committed fixture, when present)
-    fixture = os.path.join(ROOT, "tests", "golden", "h2o64_sto3g_energy.json")
-    if os.path.abspath(args.xyz).endswith("h2o_64.xyz") and os.path.isfile(fixture):
-        with open(fixture) as fh:
-            e_cpu = json.load(fh)["energy"]
-        out["energy_cpu_oracle"] = e_cpu
-        out["dE_fp64_vs_cpu_oracle"] = e64 - e_cpu
-        out["dE_mixed_vs_cpu_oracle"] = emx - e_cpu
     if not args.no_reference_inputs:
         out["reference_inputs"] = reference_inputs(pyfacc, common)
         out["gradient_h2o32"] = gradient_timing(pyfacc, common)

@@ -18,7 +18,10 @@ Every number is written with repr(), so the JSON holds it exactly.  Run from
 the repository root: python tests/golden/make_oracle_fixtures.py (a few
 minutes on 8 cores, most of it the (H2O)16 part).  With --h2o16-sto6g it
 only adds the full (H2O)16 cluster in STO-6G (two passes over its 6.8e9
-primitive quartets: about half an hour on 8 cores) to the existing file."""
+primitive quartets: about an hour on 8 cores) to the existing file; with
+--h2o64-jk it only adds checksums of J and K of the (H2O)64 cluster
+(h2o_64.xyz, 448 basis functions, STO-3G) at a seeded density, one direct
+build screened at Q_ab Q_cd < 1e-12 (a few minutes)."""
 import json
 import os
 import sys
@@ -102,6 +105,19 @@ def cluster_entry(nw, basis):
 
 def main():
     path = os.path.join(HERE, "oracle_fixtures.json")
+    if "--h2o64-jk" in sys.argv:
+        with open(path) as fh:
+            fx = json.load(fh)
+        sp, xyz = load_xyz("h2o_64.xyz")
+        m = oracle.Molecule(sp, xyz)
+        D = seeded_density(m.nbf, 64)
+        J, K, done = m.jk_direct(D, thresh=1e-12)
+        fx["h2o64_sto3g_jk"] = {"basis": "sto-3g", "nbf": m.nbf, "density_seed": 64, "schwarz": 1e-12,
+                                "quartets": int(done), "D": checksums(D), "J": checksums(J), "K": checksums(K)}
+        print("h2o64_sto3g_jk", fx["h2o64_sto3g_jk"]["J"], flush=True)
+        with open(path, "w") as fh:
+            json.dump(fx, fh)
+        return
     if "--h2o16-sto6g" in sys.argv:
         with open(path) as fh:
             fx = json.load(fh)

@@ -105,3 +105,26 @@ def test_water_cluster_jk_checksums_and_energy_vs_fixture(gpu_lib, fx, key):
     r = pyfacc.scf(sp, xyz, 5 * nw, 5 * nw, basis=e["basis"], eps=1e-10, max_iter=200, diis=8, schwarz=1e-12)
     assert r["converged"]
     assert abs(r["energy"] - e["energy"]) < 1e-9
+
+
+def test_h2o64_jk_checksums_vs_fixture(gpu_lib, fx):
+    """The Fock build at the headline cluster's full size: direct J/K of
+    (H2O)64 STO-3G (448 basis functions) at a seeded density against the
+    oracle's (which drops quartets with Q_ab Q_cd < 1e-12; the GPU screens at
+    Q_ab Q_cd max|D| < 1e-14): FP64 checksums within 1e-9 relative, FP32
+    within 1e-5."""
+    import pyfacc
+    if "h2o64_sto3g_jk" not in fx:
+        pytest.skip("h2o64_sto3g_jk not in the fixture file (make_oracle_fixtures.py --h2o64-jk adds it)")
+    e = fx["h2o64_sto3g_jk"]
+    sp, xyz = load_xyz("h2o_64.xyz")
+    D = _seeded_density(e["nbf"], e["density_seed"])
+    for k, v in _checksums(D).items():
+        assert abs(v - e["D"][k]) <= 1e-13 * abs(e["D"][k])
+    for fp32, tol in ((False, 1e-9), (True, 1e-5)):
+        J, K = pyfacc.jk(sp, xyz, D, schwarz=1e-14, fp32=fp32)
+        for M, name in ((J, "J"), (K, "K")):
+            got = _checksums(M)
+            for k in ("sum", "fro", "signed"):
+                ref = e[name][k]
+                assert abs(got[k] - ref) <= tol * max(abs(ref), e[name]["fro"]), (fp32, name, k)
