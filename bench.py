@@ -106,18 +106,20 @@ def cpu_baseline(species, xyz, n_builds, target_s):
 
 def reference_inputs(pyfacc, common):
     """Mixed and FP64 SCFs of two of the reference's own inputs (a 60-residue
-    glycine chain, 1387 bf; an RNA fragment with P, 491 bf), one untimed run
-    each first.  The RNA fragment uses the orthonormal DIIS error."""
+    glycine chain, 1387 bf; an RNA fragment with P, 491 bf) and of the UHF
+    doublet (H2O)15OH (111 bf), one untimed run each first.  The RNA fragment
+    uses the orthonormal DIIS error."""
     res = {}
-    for name, extra in (("gly60.xyz", {}), ("rna1.xyz", {"diis_basis": 1})):
+    for name, extra in (("gly60.xyz", {}), ("rna1.xyz", {"diis_basis": 1}), ("h2o_15oh.xyz", {})):
         sp, xyz = load_xyz(os.path.join(ROOT, "pyfacc-scf_amd", "data", "molecules", name))
-        nocc = sum(pyfacc.helper_functions.periodic_table[s] for s in sp) // 2
-        row = {"natoms": len(sp)}
+        nel = sum(pyfacc.helper_functions.periodic_table[s] for s in sp)
+        nocc, nbeta = (nel + 1) // 2, nel // 2  # the (H2O)15OH doublet runs UHF
+        row = {"natoms": len(sp), "uhf": nocc != nbeta}
         for form, prec in (("mixed", 1), ("fp64", 0)):
             o = pyfacc.default_options(precision=prec, **dict(common, max_iter=150, **extra))
-            pyfacc.scf(sp, xyz, nocc, nocc, options=o)
+            pyfacc.scf(sp, xyz, nocc, nbeta, options=o)
             t0 = time.perf_counter()
-            r = pyfacc.scf(sp, xyz, nocc, nocc, options=o)
+            r = pyfacc.scf(sp, xyz, nocc, nbeta, options=o)
             row[f"scf_time_{form}_s"] = time.perf_counter() - t0
             row[f"iterations_{form}"] = r["iterations"]
             row[f"converged_{form}"] = bool(r["converged"])
