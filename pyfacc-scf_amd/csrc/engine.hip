@@ -85,6 +85,8 @@ Engine::~Engine() {
         (void)hipStreamDestroy(st_);
     }
     for (auto &b : jbuf_) pool_release(b);  // the stream is idle: nothing pending uses them
+    for (auto &b : pending_)                  // class timings of builds never collected (an SCF that threw)
+        for (auto &e : b.ev) event_release(e);
 }
 
 void Engine::setup_one_electron() {
@@ -312,7 +314,11 @@ unsigned long long Engine::collect_profile() {
             prof_prims[6 * p + q] += c[12 * p + 6 + q];
             total += c[12 * p + q];
         }
-    for (auto &b : pending_) {
+    // taken out of pending_ first: a failed read below leaves its events
+    // unreleased rather than released twice (by the destructor as well)
+    std::vector<PendingBuild> done;
+    done.swap(pending_);
+    for (auto &b : done) {
         for (int q = 0; q < 6; ++q) {
             float ms = 0.f;
             PF_HIP(hipEventElapsedTime(&ms, b.ev[2 * q], b.ev[2 * q + 1]));
@@ -320,7 +326,6 @@ unsigned long long Engine::collect_profile() {
         }
         for (auto &e : b.ev) event_release(e);
     }
-    pending_.clear();
     return total;
 }
 
@@ -562,6 +567,13 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     // per iteration: events at the start of the Fock build, its end, and the end
     // of the density/energy step (t_fock and t_diag without host round trips)
     std::vector<std::array<hipEvent_t, 3>> tev;
+    struct ReleaseEvents {  // back to the pool on every exit, also when the SCF throws
+        std::vector<std::array<hipEvent_t, 3>> &v;
+        ~ReleaseEvents() {
+            for (auto &a : v)
+                for (auto e : a) event_release(e);
+        }
+    } release_tev{tev};
     // DIIS error in the orthonormal basis on request, e' = X^T e X (in place; w1_
     // as scratch): the AO commutator weights each element by the overlap metric;
     // on large molecules with small gaps the orthonormal form can take fewer
@@ -755,7 +767,6 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         PF_HIP(hipEventElapsedTime(&b, e3[1], e3[2]));
         res.t_fock += 1e-3 * a;
         res.t_diag += 1e-3 * b;
-        for (auto &e : e3) event_release(e);
     }
     res.iterations = std::min(it, opt.max_iter);
     res.converged = conv ? 1 : 0;
