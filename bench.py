@@ -92,6 +92,21 @@ def cpu_baseline(species, xyz, n_builds, target_s):
     stride = max(2, int(np.ceil(full_est / target_s)))
     t_cal, done = timed(stride)
     t_build = max(t_cal - t_loop, 1e-9) * stride + t_loop
+    # the same build on one thread, on a sparser sample (about target_s / 4);
+    # its pair-loop overhead is taken as the all-thread one times the threads
+    oracle.set_threads(1)
+    try:
+        stride1 = max(2, int(np.ceil(t_build * threads / max(target_s / 4, 1.0))))
+        t1, done1 = timed(stride1)
+    finally:
+        oracle.set_threads(threads)
+    t_build1 = max(t1 - t_loop * threads, 1e-9) * stride1 + t_loop * threads
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": 1.0 / (t_build * n_builds),
         "unit": "SCF/s",
@@ -101,6 +116,10 @@ def cpu_baseline(species, xyz, n_builds, target_s):
                    f"screened shell quartet ({done} quartets, {t_cal:.1f} s); full build = {t_build:.1f} s, "
                    f"x {n_builds} builds of the FP64 GPU SCF"),
         "fock_build_s": t_build,
+        "host": {"nproc": os.cpu_count(), "cpu_model": model},
+        "one_thread": {"value": 1.0 / (t_build1 * n_builds), "fock_build_s": t_build1,
+                       "sample": f"every {stride1}-th screened shell quartet ({done1} quartets, {t1:.1f} s)",
+                       "parallel_speedup": t_build1 / t_build},
     }
 
 
