@@ -391,14 +391,28 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
         }
         return;
     }
-    if (opt.diag == PF_DIAG_JACOBI || eps_dev) {
+    auto by_eigenvectors = [&]() {
         DevBuf<double> w(n), C(size_t(n) * n);
         jacobi_eigh(n, w2_.p, eps_dev ? eps_dev : w.p, C.p, st_);
         gemm_f64(n, false, false, 1.0, X_.p, C.p, 0.0, w1_.p, st_);  // C = X C'
         gemm_f64_mnk(n, n, nocc, false, true, occ, w1_.p, n, w1_.p, n, 0.0, D, n, st_);
+    };
+    if (opt.diag == PF_DIAG_JACOBI || eps_dev) {
+        by_eigenvectors();
         return;
     }
-    sp2_hint_ = sp2_density(n, w2_.p, double(nocc), w3_.p, w1_.p, state_.p, st_, 100, sp2_hint_ + 1);
+    constexpr int SP2_MAX = 100;
+    const int used = sp2_density(n, w2_.p, double(nocc), w3_.p, w1_.p, state_.p, st_, SP2_MAX, sp2_hint_ + 1);
+    if (used >= SP2_MAX) {
+        // No projector: the occupied/virtual boundary falls inside a degenerate
+        // level (an open shell from a spin-symmetric start, such as the pi pair
+        // of OH), where purification leaves fractional occupations and only
+        // summation-order noise would break the symmetry.  The eigenvector
+        // route occupies whole orbitals.
+        by_eigenvectors();
+        return;
+    }
+    sp2_hint_ = used;
     gemm_f64(n, false, false, 1.0, X_.p, w3_.p, 0.0, w1_.p, st_);       // X P
     gemm_f64(n, false, true, occ, w1_.p, X_.p, 0.0, D, st_);            // occ X P X^T
 }
