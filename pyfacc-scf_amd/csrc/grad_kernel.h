@@ -79,6 +79,12 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
     const PairView<double> &K = g.ket;
     if (i >= B.n) return;
     const int lane = threadIdx.x;
+    // blockIdx.y takes one slice of the ket list (slices of a multiple of 64
+    // covering all K.n kets), so a class with few bra pairs still fills the GPU;
+    // same-class lists end at ket i, so later slices have nothing to do
+    const int slice = ((K.n + (int)gridDim.y - 1) / (int)gridDim.y + 63) & ~63;
+    const int jbeg = (int)blockIdx.y * slice;
+    if (jbeg >= K.n || (g.same && jbeg > i)) return;
     const double Qi = B.Q[i];
     const double dm = g.dmax ? *g.dmax : 1.0;
     const double qthr = g.tau / (Qi * dm * dm);
@@ -89,10 +95,7 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
         else hi = mid;
     }
     const int nket = g.same ? min(lo, i + 1) : lo;
-    // blockIdx.y takes one slice of the ket list (slices of a multiple of 64
-    // covering all K.n kets), so a class with few bra pairs still fills the GPU
-    const int slice = ((K.n + (int)gridDim.y - 1) / (int)gridDim.y + 63) & ~63;
-    const int jbeg = (int)blockIdx.y * slice, jend = min(nket, jbeg + slice);
+    const int jend = min(nket, jbeg + slice);
     if (jbeg >= jend) return;
     const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
     const double ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
