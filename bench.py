@@ -124,16 +124,19 @@ def cpu_baseline(species, xyz, n_builds, target_s):
 
 
 def reference_inputs(pyfacc, common):
-    """Mixed and FP64 SCFs of two of the reference's own inputs (a 60-residue
-    glycine chain, 1387 bf; an RNA fragment with P, 491 bf) and of the UHF
-    doublet (H2O)15OH (111 bf), one untimed run each first.  The RNA fragment
-    uses the orthonormal DIIS error."""
+    """Mixed and FP64 SCFs of three of the reference's own inputs (a
+    60-residue glycine chain, 1387 bf; RNA fragments of 4 and 16 nucleotides,
+    491 and 1943 bf, as phosphate anions) and of the UHF doublet (H2O)15OH
+    (111 bf), one untimed run each first."""
     res = {}
-    for name, extra in (("gly60.xyz", {}), ("rna1.xyz", {"diis_basis": 1}), ("h2o_15oh.xyz", {})):
+    for name, anion, extra in (("gly60.xyz", False, {}), ("rna1.xyz", True, {}), ("rna4.xyz", True, {}),
+                               ("h2o_15oh.xyz", False, {})):
         sp, xyz = load_xyz(os.path.join(ROOT, "pyfacc-scf_amd", "data", "molecules", name))
-        nel = sum(pyfacc.helper_functions.periodic_table[s] for s in sp)
+        # the RNA fragments carry one negative charge per phosphate
+        charge = -sp.count("P") if anion else 0
+        nel = sum(pyfacc.helper_functions.periodic_table[s] for s in sp) - charge
         nocc, nbeta = (nel + 1) // 2, nel // 2  # the (H2O)15OH doublet runs UHF
-        row = {"natoms": len(sp), "uhf": nocc != nbeta}
+        row = {"natoms": len(sp), "charge": charge, "uhf": nocc != nbeta}
         for form, prec in (("mixed", 1), ("fp64", 0)):
             o = pyfacc.default_options(precision=prec, **dict(common, max_iter=150, **extra))
             pyfacc.scf(sp, xyz, nocc, nbeta, options=o)

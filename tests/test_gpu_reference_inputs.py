@@ -56,11 +56,15 @@ def _density_checks(sp, xyz, D, nelec):
     assert np.abs(D @ S @ D - 2 * D).max() < 1e-8
 
 
-@pytest.mark.parametrize("name,opts", [("gly30.xyz", {}), ("rna1.xyz", {"diis_basis": 1})])
-def test_reference_molecules_converge_mixed_and_fp64(gpu_lib, name, opts):
+@pytest.mark.parametrize("name,charge,opts", [("gly30.xyz", 0, {}), ("rna1.xyz", 0, {"diis_basis": 1}),
+                                              ("rna4.xyz", -16, {})])
+def test_reference_molecules_converge_mixed_and_fp64(gpu_lib, name, charge, opts):
+    """rna1 neutral (the count the reference's run_scf uses by default) with
+    the orthonormal DIIS error; rna4 with one negative charge per phosphate
+    (neutral, its gap closes and the SCF does not converge)."""
     import pyfacc
     sp, xyz = load_xyz(name)
-    ne = int(sum(pyfacc.helper_functions.periodic_table[s] for s in sp))
+    ne = int(sum(pyfacc.helper_functions.periodic_table[s] for s in sp)) - charge
     kw = dict(eps=1e-8, max_iter=150, guess=1, **opts)
     f = pyfacc.scf(sp, xyz, ne // 2, ne // 2, want_density=True, **kw)
     m = pyfacc.scf(sp, xyz, ne // 2, ne // 2, precision=1, **kw)
