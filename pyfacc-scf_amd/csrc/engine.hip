@@ -121,8 +121,8 @@ void Engine::setup_pairs() {
             all[sh[f].l + sh[s].l].push_back({f, s});
         }
     double bound_max = 0.0;
-    auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list, double kref) {
-        fill_pairset(ps, sh, d_shells_.p, list, kref, bound_max, true, st_);
+    auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list, double kref, bool want32) {
+        fill_pairset(ps, sh, d_shells_.p, list, kref, bound_max, want32, st_);
     };
     const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
     // pass 1: all pairs, Schwarz bounds
@@ -130,7 +130,7 @@ void Engine::setup_pairs() {
     for (int c = 0; c < 3; ++c) {
         PairSet &ps = pairs_[c];
         ps.la = cls_l[c][0]; ps.lb = cls_l[c][1];
-        fill(ps, all[c], -1.0);
+        fill(ps, all[c], -1.0, false);  // FP64 only: the bounds need no FP32 copy
         ps.d_Q.alloc(std::max(ps.n, 1));
         launch_schwarz<double>(ps.la, ps.lb, ps.view64(), d_tab_.p, ps.d_Q.p, st_);
         Qall[c].resize(ps.n);
@@ -141,15 +141,22 @@ void Engine::setup_pairs() {
     for (auto &q : Qall) for (double v : q) qmax = std::max(qmax, v);
     // pass 2: keep pairs that can pass Q_ab Q_max >= tau, sorted by Q descending
     for (int c = 0; c < 3; ++c) {
-        std::vector<int> idx;
+        // (Q, index) sorted by Q descending, ties by index: the order of a stable
+        // sort, without its indirect comparisons
+        std::vector<std::pair<double, int>> key;
+        key.reserve(Qall[c].size());
         for (int i = 0; i < (int)Qall[c].size(); ++i)
-            if (Qall[c][i] * qmax >= schwarz_) idx.push_back(i);
-        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return Qall[c][x] > Qall[c][y]; });
+            if (Qall[c][i] * qmax >= schwarz_) key.push_back({Qall[c][i], i});
+        std::sort(key.begin(), key.end(), [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+            return x.first > y.first || (x.first == y.first && x.second < y.second);
+        });
         std::vector<std::pair<int, int>> kept;
         std::vector<double> q;
-        for (int i : idx) { kept.push_back(all[c][i]); q.push_back(Qall[c][i]); }
+        kept.reserve(key.size());
+        q.reserve(key.size());
+        for (const auto &k : key) { kept.push_back(all[c][k.second]); q.push_back(k.first); }
         PairSet &ps = pairs_[c];
-        fill(ps, kept, bound_max);
+        fill(ps, kept, bound_max, true);
         ps.Q = q;
         ps.d_Q.upload(q.data(), q.size(), st_);
         std::vector<float> fq(q.begin(), q.end());
