@@ -15,6 +15,8 @@
 
 namespace pf {
 
+constexpr int GRAD1E_ATOMS = 4;  // nuclei per workgroup of the attraction term
+
 template <int LA, int LB>
 __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ shells, const int *__restrict__ pa,
                                                      const int *__restrict__ pb, int npairs, int nbf, int natoms,
@@ -42,8 +44,13 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
     for (int d = 0; d < 3; ++d) AB[d] = A.xyz[d] - B.xyz[d];
     const double r2ab = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
     double gownT[3] = {0, 0, 0}, gownV[3] = {0, 0, 0}, gownS[3] = {0, 0, 0};
+    // blockIdx.y takes GRAD1E_ATOMS nuclei of the attraction term; the overlap
+    // and kinetic terms belong to the first chunk
+    const int cbeg = (int)blockIdx.y * GRAD1E_ATOMS, cend = min(natoms, cbeg + GRAD1E_ATOMS);
+    const bool first = blockIdx.y == 0;
 
     // ---- overlap and kinetic
+    if (first)
     for (int i = 0; i < A.np; ++i)
         for (int j = 0; j < B.np; ++j) {
             const double a = A.exp[i], b = B.exp[j], p = a + b, ip = 1.0 / p, o2p = 0.5 * ip;
@@ -107,7 +114,7 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
         }
 
     // ---- nuclear attraction, one nucleus at a time (wavefront-uniform loop)
-    for (int c = 0; c < natoms; ++c) {
+    for (int c = cbeg; c < cend; ++c) {
         double val[3] = {0, 0, 0};
         const double zc = -double(Z[c]);
         for (int i = 0; i < A.np; ++i)
@@ -168,9 +175,11 @@ __global__ void __launch_bounds__(64) grad1e_kernel(const Shell *__restrict__ sh
     if (active)
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            atomicAdd(&gT[3 * A.atom + x], gownT[x]);
+            if (first) {
+                atomicAdd(&gT[3 * A.atom + x], gownT[x]);
+                atomicAdd(&gS[3 * A.atom + x], gownS[x]);
+            }
             atomicAdd(&gV[3 * A.atom + x], gownV[x]);
-            atomicAdd(&gS[3 * A.atom + x], gownS[x]);
         }
 }
 
@@ -178,7 +187,9 @@ void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int
                    const double *axyz, const int *Z, const double *tab, const double *D, const double *W,
                    double *gT, double *gV, double *gS, hipStream_t st) {
     if (npairs == 0) return;
-    const dim3 grid((npairs + 63) / 64), block(64);
+    // (pair block, chunk of nuclei): one thread per pair alone leaves a
+    // small class at a few wavefronts for the whole chip
+    const dim3 grid((npairs + 63) / 64, std::max(1, (natoms + GRAD1E_ATOMS - 1) / GRAD1E_ATOMS)), block(64);
     if (la == 0 && lb == 0) grad1e_kernel<0, 0><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
     else if (la == 0 && lb == 1) grad1e_kernel<0, 1><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
     else if (la == 1 && lb == 0) grad1e_kernel<1, 0><<<grid, block, 0, st>>>(shells, pa, pb, npairs, nbf, natoms, axyz, Z, tab, D, W, gT, gV, gS);
