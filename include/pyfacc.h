@@ -75,10 +75,10 @@ typedef struct pf_options {
                              superposition of spherically averaged atomic UHF densities */
     double level_shift;   /* Eh added to the virtual space, F + b (S - S D S / occ), after DIIS extrapolation,
                              while ||dD||_F of the previous iteration is >= level_shift_until (0: off;
-                             pf_scf and pf_scf_gradient, not pf_batch_scf) */
+                             pf_scf and pf_scf_gradient; pf_batch_scf rejects a nonzero value) */
     double level_shift_until;
     int diis_basis;       /* DIIS error vector: PF_DIIS_AO, F D S - S D F; PF_DIIS_ORTHO, X^T (F D S - S D F) X
-                             with X the orthogonaliser (pf_scf and pf_scf_gradient) */
+                             with X the orthogonaliser (pf_scf and pf_scf_gradient; pf_batch_scf: AO only) */
     double mixed_anchor;  /* mixed precision: the full FP64 rebuild inside the FP32 phase comes when ||dD||_F
                              first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient) */
 } pf_options;

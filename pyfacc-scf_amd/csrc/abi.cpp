@@ -218,6 +218,8 @@ int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int
         else pf_default_options(&o);
         if (o.max_iter < 1) throw Error("max_iter must be >= 1");
         if (nmol <= 0) throw Error("nmol must be positive");
+        if (o.level_shift != 0.0 || o.diis_basis != PF_DIIS_AO)
+            throw Error("level_shift and diis_basis are not available in the batched engine (pf_batch_scf)");
         if (!natoms || !nalpha || !nbeta) throw Error("null natoms, nalpha or nbeta");
         std::vector<BatchMolecule> mols(nmol);
         size_t a0 = 0;

@@ -121,3 +121,13 @@ def test_flop_model_counts():
     assert flops.per_primitive(1, 1, 1, 1) > flops.per_primitive(1, 1, 1, 0) > flops.per_primitive(1, 0, 0, 0)
     assert flops.fock_flops([1, 0, 0, 0, 0, 0], kprim_pair=9) == 81 * flops.per_primitive(0, 0, 0, 0) + \
         flops.per_contracted(0, 0, 0, 0)
+
+
+def test_batch_rejects_single_molecule_accelerators():
+    """The batched engine has no level shift or orthonormal DIIS error; asking
+    for them is an input error (raised before any device work)."""
+    import pyfacc
+    mol = (["H", "H"], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]]), 1, 1)
+    for bad in ({"level_shift": 0.5}, {"diis_basis": 1}):
+        with pytest.raises(pyfacc.PyfaccError, match="batched engine"):
+            pyfacc.batch_scf([mol], **bad)
