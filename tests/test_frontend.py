@@ -131,3 +131,21 @@ def test_batch_rejects_single_molecule_accelerators():
     for bad in ({"level_shift": 0.5}, {"diis_basis": 1}):
         with pytest.raises(pyfacc.PyfaccError, match="batched engine"):
             pyfacc.batch_scf([mol], **bad)
+
+
+def test_options_layout_matches_the_library():
+    """pf_default_options fills the ctypes mirror of pf_options: every field
+    reads back the documented default (a field added on one side only would
+    shift those after it), and the struct ends where the last field does."""
+    import ctypes
+    import pyfacc
+    o = pyfacc._native.default_options()
+    expect = dict(eps=1e-8, max_iter=100, precision=0, incremental=0, eri_mode=0, diis=8, diag=0, renormalize=1,
+                  schwarz=1e-10, mixed_switch=1e-5, verbose=0, schwarz_inc=1e-12, rebuild_every=0,
+                  schwarz_fp32=1e-6, guess=0, level_shift=0.0, level_shift_until=0.1, diis_basis=0,
+                  mixed_anchor=0.1)
+    assert [f for f, _ in pyfacc._native.Options._fields_] == list(expect)
+    for k, v in expect.items():
+        assert getattr(o, k) == v, k
+    last = pyfacc._native.Options._fields_[-1][0]
+    assert ctypes.sizeof(o) == getattr(pyfacc._native.Options, last).offset + ctypes.sizeof(ctypes.c_double)
