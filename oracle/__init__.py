@@ -225,17 +225,24 @@ class Molecule:
                                   _p(J), _p(K), ctypes.c_int64(stride))
         return J, K, done
 
-    def scf(self, nalpha, nbeta, eps=1e-10, max_iter=100, eri_mode=0, schwarz=0.0, diis=0):
+    def scf(self, nalpha, nbeta, eps=1e-10, max_iter=100, eri_mode=0, schwarz=0.0, diis=0, force_uhf=False):
+        """RHF when nalpha == nbeta, else UHF (uscf.f90:65-213); force_uhf takes
+        the UHF branch for a closed shell too (two spin densities)."""
         class _Res(ctypes.Structure):
             _fields_ = [("energy", ctypes.c_double), ("e_elec", ctypes.c_double), ("e_nuc", ctypes.c_double),
                         ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("fock_builds", ctypes.c_int)]
         r = _Res()
-        nd = 1 if nalpha == nbeta else 2
+        nd = 1 if (nalpha == nbeta and not force_uhf) else 2
         D = np.zeros((nd, self.nbf, self.nbf))
         eps_out = np.zeros((nd, self.nbf))
-        lib().or_scf(*self.basis.args(), ctypes.c_int(len(self.Z)), _pi(self.Z), _p(self.xyz), ctypes.c_int(nalpha),
-                     ctypes.c_int(nbeta), ctypes.c_double(eps), ctypes.c_int(max_iter), ctypes.c_int(eri_mode),
-                     ctypes.c_double(schwarz), ctypes.c_int(diis), ctypes.byref(r), _p(D), _p(eps_out))
+        lib().or_set_force_uhf(ctypes.c_int(1 if force_uhf else 0))
+        try:
+            lib().or_scf(*self.basis.args(), ctypes.c_int(len(self.Z)), _pi(self.Z), _p(self.xyz),
+                         ctypes.c_int(nalpha), ctypes.c_int(nbeta), ctypes.c_double(eps), ctypes.c_int(max_iter),
+                         ctypes.c_int(eri_mode), ctypes.c_double(schwarz), ctypes.c_int(diis), ctypes.byref(r), _p(D),
+                         _p(eps_out))
+        finally:
+            lib().or_set_force_uhf(ctypes.c_int(0))
         return dict(energy=r.energy, e_elec=r.e_elec, e_nuc=r.e_nuc, iterations=r.iterations,
                     converged=bool(r.converged), fock_builds=r.fock_builds, D=D if nd == 2 else D[0],
                     eps=eps_out)

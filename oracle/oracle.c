@@ -47,6 +47,10 @@ static const double OR_PI = 3.14159265358979323846;
 static int g_compat_boys = 0;
 
 void or_set_compat_boys(int on) { g_compat_boys = on; }
+/* 1: or_scf takes the UHF branch (two spin densities, uscf.f90:149-203) even
+ * when nalpha == nbeta, so that it can be held to the RHF branch. */
+static int g_force_uhf = 0;
+void or_set_force_uhf(int on) { g_force_uhf = on; }
 
 /* ------------------------------------------------------------------ */
 /* Boys function                                                        */
@@ -814,7 +818,7 @@ int or_scf(int nsh, const int *l, const int *np, const double *ctr, const double
     shell_offsets(&bs, off, &n);
     free(off);
     size_t n2 = (size_t)n * n;
-    int uhf = nalpha != nbeta, nd = uhf ? 2 : 1;
+    int uhf = nalpha != nbeta || g_force_uhf, nd = uhf ? 2 : 1;
     double *S = malloc(sizeof(double) * n2), *T = malloc(sizeof(double) * n2), *V = malloc(sizeof(double) * n2);
     double *H = malloc(sizeof(double) * n2), *X = malloc(sizeof(double) * n2);
     double *D = malloc(sizeof(double) * n2 * 2), *Dold = malloc(sizeof(double) * n2 * 2);

@@ -250,13 +250,98 @@ def test_size_consistency():
     assert e2 == pytest.approx(2 * e1, abs=1e-9)
 
 
-def test_uhf_oh_and_spin_symmetric_limit():
+# ---------------------------------------------------------------- UHF pins
+# The oracle's UHF branch (uscf.f90:149-203: G_s = J[Da + Db] - K[D_s],
+# E = 1/2 sum (Da + Db) H + 1/2 sum (Da Fa + Db Fb)) held to what the
+# mathematics fixes: one-electron systems, the closed-shell limit, a
+# two-electron triplet whose determinant the basis fixes, size-consistency
+# and the UHF stationarity conditions.
+def _one_electron_exact(m):
+    """Lowest generalised eigenvalue of (T + V, S) plus E_nuc: the exact
+    energy of a one-electron system in the basis (no self-interaction)."""
+    import scipy.linalg
+    S, T, V = m.one_electron()
+    return scipy.linalg.eigh(T + V, S, eigvals_only=True)[0] + m.enuc()
+
+
+@pytest.mark.parametrize("name,species,xyz", [
+    ("H", ["H"], [[0.0, 0.0, 0.0]]),
+    ("H2+", ["H", "H"], [[0.0, 0.0, 0.0], [0.0, 0.0, 2.0]]),
+    ("He+", ["He"], [[0.0, 0.0, 0.0]]),
+    ("H3++", ["H", "H", "H"], [[0.0, 0.0, 0.0], [1.8, 0.0, 0.0], [0.6, 1.5, 0.3]]),
+    ("Li++", ["Li"], [[0.0, 0.0, 0.0]]),
+    ("LiH+++", ["Li", "H"], [[0.0, 0.0, 0.0], [0.4, 0.9, 2.6]]),
+])
+@pytest.mark.parametrize("basis", ["sto-3g", "sto-6g"])
+def test_uhf_one_electron_systems_are_exact(name, species, xyz, basis):
+    """nalpha = 1, nbeta = 0: J[Da] = K[Da] for a single occupied orbital, so
+    F_a = H and E is the lowest eigenvalue of H c = e S c.  A wrong exchange
+    factor or energy weight leaves self-interaction and fails this."""
+    m = oracle.Molecule(species, np.array(xyz), basis)
+    r = m.scf(1, 0, eps=1e-12, max_iter=200)
+    assert r["converged"] and r["D"].shape == (2, m.nbf, m.nbf)
+    assert r["energy"] == pytest.approx(_one_electron_exact(m), abs=1e-12), name
+    assert np.abs(r["D"][1]).max() == 0.0
+
+
+def test_forced_uhf_reproduces_rhf():
+    """A closed shell through the UHF branch: same energy and Da = Db = D_RHF / 2."""
+    sp, xyz = load_xyz("h2o_c1.xyz")
+    m = oracle.Molecule(sp, xyz)
+    rr = m.scf(5, 5, eps=1e-11, max_iter=200)
+    ru = m.scf(5, 5, eps=1e-11, max_iter=200, force_uhf=True)
+    assert rr["converged"] and ru["converged"] and ru["D"].shape == (2, m.nbf, m.nbf)
+    assert ru["energy"] == pytest.approx(rr["energy"], abs=1e-10)
+    assert np.abs(ru["D"][0] - 0.5 * rr["D"]).max() < 1e-8
+    assert np.abs(ru["D"][1] - 0.5 * rr["D"]).max() < 1e-8
+    assert ru["iterations"] == rr["iterations"]
+
+
+def test_uhf_triplet_slater_condon():
+    """H2 STO-3G with nalpha = 2, nbeta = 0: both alpha orbitals of the
+    two-function basis are occupied, so the determinant is fixed and its
+    energy is the Slater-Condon sum over any orthonormal basis of the space
+    (here Loewdin orbitals), E = h_11 + h_22 + (11|22) - (12|21) + E_nuc."""
+    m = oracle.Molecule(["H", "H"], np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 1.45]]))
+    r = m.scf(2, 0, eps=1e-12, max_iter=50)
+    assert r["converged"]
+    S, T, V = m.one_electron()
+    w, U = np.linalg.eigh(S)
+    C = U @ np.diag(w ** -0.5) @ U.T  # Loewdin orbitals (columns)
+    h = C.T @ (T + V) @ C
+    g = np.einsum("pqrs,pi,qj,rk,sl->ijkl", m.eri_full(), C, C, C, C)
+    e = h[0, 0] + h[1, 1] + g[0, 0, 1, 1] - g[0, 1, 1, 0] + m.enuc()
+    assert r["energy"] == pytest.approx(e, abs=1e-12)
+    assert np.abs(r["D"][0] - np.linalg.inv(S)).max() < 1e-10
+
+
+def test_uhf_oh_stationary_idempotent():
+    """OH doublet: the oracle's converged spin densities are idempotent with the
+    right traces and commute with Fock matrices built independently from the
+    full ERI tensor (G_a = J[Da + Db] - K[Da])."""
     sp, xyz = load_xyz("oh.xyz")
-    r = oracle.Molecule(sp, xyz).scf(5, 4, eps=1e-10, max_iter=300)
+    m = oracle.Molecule(sp, xyz)
+    r = m.scf(5, 4, eps=1e-11, max_iter=300, diis=6)
     assert r["converged"] and r["D"].shape[0] == 2
-    sp2, xyz2 = load_xyz("h2o_c1.xyz")
-    m = oracle.Molecule(sp2, xyz2)
-    # UHF code path with na == nb reproduces RHF: run the UHF driver through a
-    # closed-shell cation-free check (same energy within convergence)
-    rr = m.scf(5, 5, eps=1e-10, max_iter=200)
-    assert rr["converged"]
+    S, T, V = m.one_electron()
+    H = T + V
+    eri = m.eri_full()
+    Da, Db = r["D"]
+    J = np.einsum("pqrs,rs->pq", eri, Da + Db)
+    for Ds, nocc in ((Da, 5), (Db, 4)):
+        F = H + J - np.einsum("prqs,rs->pq", eri, Ds)
+        assert np.trace(Ds @ S) == pytest.approx(nocc, abs=1e-10)
+        assert np.abs(Ds @ S @ Ds - Ds).max() < 1e-9
+        assert np.abs(F @ Ds @ S - S @ Ds @ F).max() < 1e-8
+
+
+def test_uhf_size_consistency():
+    """OH + H2O 5000 bohr apart: E(UHF, doublet) = E_UHF(OH) + E_RHF(H2O)."""
+    sp1, x1 = load_xyz("oh.xyz")
+    sp2, x2 = load_xyz("h2o_c1.xyz")
+    e1 = oracle.Molecule(sp1, x1).scf(5, 4, eps=1e-11, max_iter=300, diis=6)
+    e2 = oracle.Molecule(sp2, x2).scf(5, 5, eps=1e-11, max_iter=300, diis=6)
+    far = np.vstack([x1, x2 + np.array([0.0, 0.0, 5000.0])])
+    e12 = oracle.Molecule(sp1 + sp2, far).scf(10, 9, eps=1e-11, max_iter=300, diis=6)
+    assert e1["converged"] and e2["converged"] and e12["converged"]
+    assert e12["energy"] == pytest.approx(e1["energy"] + e2["energy"], abs=1e-9)
