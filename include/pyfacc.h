@@ -81,6 +81,8 @@ typedef struct pf_options {
                              with X the orthogonaliser (pf_scf and pf_scf_gradient; pf_batch_scf: AO only) */
     double mixed_anchor;  /* mixed precision: the full FP64 rebuild inside the FP32 phase comes when ||dD||_F
                              first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient) */
+    int profile;          /* 1: time each J/K class kernel with HIP events around it (pf_scf_profile's
+                             milliseconds; the events add short gaps to the build), 0: no events */
 } pf_options;
 
 typedef struct pf_result {
@@ -106,7 +108,7 @@ typedef struct pf_result {
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
  * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error,
- * mixed_anchor 0.1. */
+ * mixed_anchor 0.1, profile 0. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */
@@ -174,7 +176,8 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
  * J/K builds, 12 entries indexed precision*6 + class (precision 0 FP64,
  * 1 FP32; classes (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)):
  * quartets that passed screening, milliseconds of the class's main ERI
- * kernel (HIP events on the engine's stream right around it: the screening
+ * kernel (HIP events on the engine's stream right around it, recorded only
+ * when pf_options.profile = 1, else 0: the screening
  * count/scan and J'_cd reduction helpers are not included), kernel launches
  * and primitive quartets evaluated (summed over quartets);
  * class_l[24] gives per class the angular momenta (la lb | lc ld) in the
@@ -194,6 +197,8 @@ typedef struct {
     int graph;          /* 1: iterations replayed from one captured hipGraph */
     int digest_waves;   /* autotuned wavefronts per molecule in the J/K digest */
     int nproblems;      /* spin densities in the batch (1 per RHF, 2 per UHF molecule) */
+    int sp2_fallbacks;  /* densities formed from eigenvectors (aufbau) because the purification did not
+                           converge: the occupied/virtual boundary inside a degenerate level */
 } pf_batch_stats;
 
 /* Batched SCF over many independent molecules in one set of launches.  No
@@ -212,6 +217,18 @@ typedef struct {
 int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int32_t *z, const double *coords,
                  const char *basis_name, const int32_t *nalpha, const int32_t *nbeta, const pf_options *opt,
                  int use_graph, double *energy, int32_t *iterations, int32_t *converged, pf_batch_stats *stats);
+
+/* Device memory kept between calls.  Every call drains its own stream, so the
+ * library caches released device blocks per calling thread for its next call
+ * (up to 8 GiB per thread); a thread that ends hands its blocks to a
+ * process-wide idle list (up to 8 GiB, reused by any thread, the rest freed).
+ * pf_release_device_cache frees the calling thread's cache, that idle list and
+ * the idle scratch blocks of finished SCFs; no library call may be running on
+ * another thread meanwhile.  pf_cached_device_bytes: bytes held by the calling
+ * thread's cache plus the idle list.  No counterpart in the reference (which
+ * allocates and frees per call); for long-running services. */
+int pf_release_device_cache(void);
+unsigned long long pf_cached_device_bytes(void);
 
 /* Device kernels exposed for testing on host buffers. */
 int pf_boys(int nmax, int npts, const double *x, double *F);               /* F[npts*(nmax+1)], nmax <= 6 */

@@ -109,6 +109,7 @@ void pf_default_options(pf_options *o) {
     o->level_shift_until = 0.1;
     o->diis_basis = PF_DIIS_AO;
     o->mixed_anchor = 0.1;
+    o->profile = 0;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }
@@ -197,7 +198,7 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
         Input in = make_input(natoms, species, z, coords);
         Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,
                    o.schwarz > 0 ? o.schwarz : 1e-14);
-        eng.profile_classes = true;
+        eng.profile_classes = o.profile != 0;
         pf_result r;
         eng.scf(nalpha, nbeta, o, r, D_out, eps_out);
         g_prof_q = eng.prof_quartets;
@@ -241,6 +242,7 @@ int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int
             stats->t_scf = bs.t_scf; stats->eri_values = bs.eri_values; stats->store_bytes = bs.store_bytes;
             stats->iterations = bs.iterations; stats->graph = bs.graph; stats->digest_waves = bs.digest_waves;
             stats->nproblems = bs.nproblems;
+            stats->sp2_fallbacks = bs.sp2_fallbacks;
         }
     });
 }
@@ -261,8 +263,11 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
                    o.schwarz > 0 ? o.schwarz : 1e-14);
         pf_result r;
         eng.scf(nalpha, nbeta, o, r, D_out, nullptr);
-        eng.gradient(grad, 1e-13, terms);
         if (res) *res = r;
+        if (!r.converged)
+            throw Error("gradient: the SCF did not converge in " + std::to_string(r.iterations) +
+                        " iterations; the analytic gradient holds only at a converged density");
+        eng.gradient(grad, 1e-13, terms);
     });
 }
 
@@ -354,6 +359,12 @@ int pf_device_copy(unsigned long long bytes, double *ms) {
         PF_HIP(hipStreamDestroy(st));
     });
 }
+
+int pf_release_device_cache(void) {
+    return guarded([&] { release_device_cache(); });
+}
+
+unsigned long long pf_cached_device_bytes(void) { return (unsigned long long)cached_device_bytes(); }
 
 void run_scf_c(int natoms, int nelectrons, int charge, int nalpha, int nbeta, const char *species, double *coords,
                double eps_scf, int max_iter, double *final_energy, const char *basis_set_c, int32_t *z_num) {

@@ -62,12 +62,15 @@ struct BatchDev {
     int *mhead = nullptr, *mnstore = nullptr, *muse = nullptr;  // DIIS ring head / size / extrapolate flag
     double *mcoef = nullptr;  // DIIS coefficients [M][ndiis]
     double *mB = nullptr;     // DIIS B matrices [M][ndiis][ndiis]
+    int *psp2fail = nullptr;  // per problem: 1 when the last purification did not converge
+    int *nfallback = nullptr; // densities formed from eigenvectors instead (whole run)
 };
 
 struct BatchStats {
     double t_total = 0, t_setup = 0, t_eri = 0, t_scf = 0;
     double eri_values = 0, store_bytes = 0;
     int iterations = 0, graph = 0, digest_waves = 0, nproblems = 0;
+    int sp2_fallbacks = 0;
 };
 
 class BatchEngine {
@@ -106,6 +109,9 @@ class BatchEngine {
     DevBuf<double> tab_;
     DevBuf<int> d_mn_, d_muhf_, d_mprob_, d_pn_, d_pmol_, d_mactive_, d_mprec_, d_mhead_, d_mnstore_, d_muse_;
     DevBuf<int> d_count_;
+    DevBuf<int> d_psp2fail_, d_nfallback_;
+    size_t jacobi_lds_ = 0;            // dynamic LDS of the eigenvector fallback
+    bool jacobi_mats_in_lds_ = false;  // its A and V in LDS (else in device memory)
     DevBuf<double> d_poccf_, d_pnocc_, d_enuc_, d_E_, d_Eold_, d_iters_, d_conv_, d_coef_, d_B_;
     DevBuf<long long> d_moff_, d_poff_, d_pmoff_, d_pdiis_;
     DevBuf<double> S_, H_, X_, Dtot_;                    // molecule pools

@@ -259,11 +259,128 @@ __global__ void __launch_bounds__(256) bsp2_kernel(BatchDev b, const double *Fp,
         cur = nxt;
         nxt = t;
     }
+    // Not a projector: the occupied/virtual boundary falls inside a degenerate
+    // level (an open shell from a spin-symmetric start), where purification
+    // leaves fractional occupations; bjacobi_density_kernel then forms this
+    // problem's density from eigenvectors (whole orbitals occupied).
+    if (threadIdx.x == 0) b.psp2fail[p] = fabs(t1 - t2) >= 1e-8 ? 1 : 0;
     double *P = Pout + o;
     for (int e = threadIdx.x; e < n * n; e += 256) {
         const int r = e / n, c = e - r * n;
         P[e] = 0.5 * (cur[e] + cur[(size_t)c * n + r]);
     }
+}
+
+// Eigenvector fallback of the purification, one workgroup per problem whose
+// bsp2_kernel flagged it: cyclic two-sided Jacobi on sym(F') with the
+// round-robin pairing of jacobi_eigh (N/2 disjoint rotations per round, each
+// 2x2 block of A and each (row, pair) of V updated in place by one thread),
+// sweeps until the off-diagonal norm is below 1e-15 of the total, then
+// P = sum over the nocc lowest eigenvalues of v v^T (ties: lower index
+// first), the aufbau density of the reference (uscf.f90:177-184).  A and V
+// live in dynamic LDS when 2 nmax^2 doubles fit (lds_mats), else in the
+// problem's slots of Ag and Vg; the rotations, eigenvalues and occupations
+// always in LDS.
+__device__ __forceinline__ void brr_pair(int N, int r, int k, int &p, int &q) {
+    if (k == 0) { p = r; q = N - 1; }
+    else { p = (r + k) % (N - 1); q = (r - k + (N - 1)) % (N - 1); }
+    if (p > q) { const int t = p; p = q; q = t; }
+}
+__global__ void __launch_bounds__(256) bjacobi_density_kernel(BatchDev b, const double *Fp, double *Ag, double *Vg,
+                                                              double *Pout, int lds_mats) {
+    extern __shared__ double jl[];
+    __shared__ double sh[4];
+    const int p = blockIdx.x, n = b.pn[p];
+    if (!b.mactive[b.pmol[p]] || !b.psp2fail[p]) return;
+    const size_t o = b.poff[p];
+    const int N = n + (n & 1), m = N / 2, Nmax = b.nmax + (b.nmax & 1);
+    const size_t nn = (size_t)b.nmax * b.nmax;
+    double *A = lds_mats ? jl : Ag + o, *V = lds_mats ? jl + nn : Vg + o;
+    double *cs = jl + (lds_mats ? 2 * nn : 0), *w = cs + Nmax, *occ = w + Nmax;
+    const double *F = Fp + o;
+    double tot = 0.0;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int r = e / n, c = e - r * n;
+        const double v = 0.5 * (F[e] + F[(size_t)c * n + r]);
+        A[e] = v;
+        V[e] = r == c ? 1.0 : 0.0;
+        tot += v * v;
+    }
+    tot = block_sum256(tot, sh);  // also orders the writes above before the reads below
+    for (int sweep = 0; sweep < 60 && n > 1; ++sweep) {
+        double off = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += 256)
+            if (e / n != e % n) off += A[e] * A[e];
+        off = block_sum256(off, sh);
+        if (off <= 1e-30 * tot) break;
+        for (int r = 0; r < N - 1; ++r) {
+            for (int k = threadIdx.x; k < m; k += 256) {
+                int pp, qq;
+                brr_pair(N, r, k, pp, qq);
+                double c = 1.0, s = 0.0;
+                if (qq < n) {
+                    const double apq = A[(size_t)pp * n + qq];
+                    if (apq != 0.0) {
+                        const double th = (A[(size_t)qq * n + qq] - A[(size_t)pp * n + pp]) / (2.0 * apq);
+                        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                    }
+                }
+                cs[k] = c;
+                cs[m + k] = s;
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < m * m; t += 256) {  // A <- G^T A G, block (k1, k2)
+                const int k1 = t / m, k2 = t - k1 * m;
+                int p1, q1, p2, q2;
+                brr_pair(N, r, k1, p1, q1);
+                brr_pair(N, r, k2, p2, q2);
+                const double c1 = cs[k1], s1 = cs[m + k1], c2 = cs[k2], s2 = cs[m + k2];
+                const bool q1ok = q1 < n, q2ok = q2 < n;
+                const double a_pp = A[(size_t)p1 * n + p2];
+                const double a_pq = q2ok ? A[(size_t)p1 * n + q2] : 0.0;
+                const double a_qp = q1ok ? A[(size_t)q1 * n + p2] : 0.0;
+                const double a_qq = (q1ok && q2ok) ? A[(size_t)q1 * n + q2] : 0.0;
+                const double b_pp = c2 * a_pp - s2 * a_pq, b_pq = s2 * a_pp + c2 * a_pq;
+                const double b_qp = c2 * a_qp - s2 * a_qq, b_qq = s2 * a_qp + c2 * a_qq;
+                double o_pp = c1 * b_pp - s1 * b_qp, o_pq = c1 * b_pq - s1 * b_qq;
+                double o_qp = s1 * b_pp + c1 * b_qp, o_qq = s1 * b_pq + c1 * b_qq;
+                if (k1 == k2) { o_pq = 0.0; o_qp = 0.0; }
+                A[(size_t)p1 * n + p2] = o_pp;
+                if (q2ok) A[(size_t)p1 * n + q2] = o_pq;
+                if (q1ok) A[(size_t)q1 * n + p2] = o_qp;
+                if (q1ok && q2ok) A[(size_t)q1 * n + q2] = o_qq;
+            }
+            for (int t = threadIdx.x; t < n * m; t += 256) {  // V <- V G, (row, pair)
+                const int row = t / m, k = t - row * m;
+                int pp, qq;
+                brr_pair(N, r, k, pp, qq);
+                const double c = cs[k], s = cs[m + k];
+                const double vp = V[(size_t)row * n + pp], vq = qq < n ? V[(size_t)row * n + qq] : 0.0;
+                V[(size_t)row * n + pp] = c * vp - s * vq;
+                if (qq < n) V[(size_t)row * n + qq] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += 256) w[i] = A[(size_t)i * n + i];
+    __syncthreads();
+    const double nocc = b.pnocc[p];
+    for (int k = threadIdx.x; k < n; k += 256) {
+        int rank = 0;
+        for (int j = 0; j < n; ++j) rank += (w[j] < w[k] || (w[j] == w[k] && j < k)) ? 1 : 0;
+        occ[k] = rank < nocc ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    double *P = Pout + o;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int r = e / n, c = e - r * n;
+        double v = 0.0;
+        for (int k = 0; k < n; ++k) v += occ[k] * V[(size_t)r * n + k] * V[(size_t)c * n + k];
+        P[e] = v;
+    }
+    if (threadIdx.x == 0) atomicAdd(b.nfallback, 1);
 }
 
 // Gershgorin discs of problem matrices (the orthogonaliser's molecule view):
@@ -505,6 +622,10 @@ void BatchEngine::setup() {
     d_enuc_.upload(enuc_.data(), M_, st_);
     d_mactive_.alloc(M_); d_mprec_.alloc(M_); d_mhead_.alloc(M_); d_mnstore_.alloc(M_); d_muse_.alloc(M_);
     d_count_.alloc(1);
+    d_psp2fail_.alloc(P_);
+    d_nfallback_.alloc(1);
+    fill_int_kernel<<<(P_ + 255) / 256, 256, 0, st_>>>(P_, 0, d_psp2fail_.p);
+    fill_int_kernel<<<1, 64, 0, st_>>>(1, 0, d_nfallback_.p);
     d_E_.alloc(M_); d_Eold_.alloc(M_); d_iters_.alloc(M_); d_conv_.alloc(M_);
     d_coef_.alloc((size_t)M_ * ndiis_); d_B_.alloc((size_t)M_ * ndiis_ * ndiis_);
     S_.alloc(mtot_); H_.alloc(mtot_); X_.alloc(mtot_); Dtot_.alloc(mtot_);
@@ -516,6 +637,24 @@ void BatchEngine::setup() {
     dev_.poff = d_poff_.p; dev_.pmoff = d_pmoff_.p; dev_.pdiis = d_pdiis_.p;
     dev_.mactive = d_mactive_.p; dev_.mprec = d_mprec_.p; dev_.mhead = d_mhead_.p; dev_.mnstore = d_mnstore_.p;
     dev_.muse = d_muse_.p; dev_.mcoef = d_coef_.p; dev_.mB = d_B_.p;
+    dev_.psp2fail = d_psp2fail_.p; dev_.nfallback = d_nfallback_.p;
+    {
+        // the fallback's matrices in LDS when they fit beside its small arrays
+        const int Nmax = nmax_ + (nmax_ & 1);
+        const size_t small = sizeof(double) * 3 * (size_t)Nmax;
+        const size_t mats = sizeof(double) * 2 * (size_t)nmax_ * nmax_;
+        int maxlds = 0, dev = 0;
+        PF_HIP(hipGetDevice(&dev));
+        PF_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t budget = (size_t)maxlds - 1024;  // static sh[4] and alignment
+        jacobi_mats_in_lds_ = small + mats <= budget;
+        const size_t dyn = jacobi_mats_in_lds_ ? small + mats : small;
+        if (dyn > budget) throw Error("batch: molecules too large for the purification fallback");
+        jacobi_lds_ = dyn;
+        if (dyn > 65536)
+            PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(bjacobi_density_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, maxlds - 1024));
+    }
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 1, d_mactive_.p);
 
     phase("basis sets and offsets");
@@ -775,6 +914,9 @@ void BatchEngine::density_step(const double *F) {
     gemm(F, d_poff_.p, 0, X_.p, d_pmoff_.p, 0, W1_.p, nullptr);      // F X
     gemm(X_.p, d_pmoff_.p, 1, W1_.p, d_poff_.p, 0, Fp_.p, nullptr);  // X^T F X
     bsp2_kernel<<<P_, 256, 0, st_>>>(dev_, Fp_.p, Pr_.p, W2_.p, W1_.p, SP2_MAX_STEPS);  // P into W1
+    // problems whose purification failed: aufbau density from eigenvectors, into W1
+    bjacobi_density_kernel<<<P_, 256, jacobi_lds_, st_>>>(dev_, Fp_.p, Pr_.p, W2_.p, W1_.p,
+                                                          jacobi_mats_in_lds_ ? 1 : 0);
     gemm(X_.p, d_pmoff_.p, 0, W1_.p, d_poff_.p, 0, W2_.p, nullptr);              // X P
     gemm(W2_.p, d_poff_.p, 0, X_.p, d_pmoff_.p, 1, D_.p, d_poccf_.p);            // occ X P X^T
     bdtot_kernel<<<g, 256, 0, st_>>>(dev_, D_.p, Dtot_.p);
@@ -892,6 +1034,8 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     std::vector<double> E(M_), iters(M_), conv(M_);
+    int nfallback = 0;
+    d_nfallback_.download(&nfallback, 1, st_);
     d_E_.download(E.data(), M_, st_);
     d_iters_.download(iters.data(), M_, st_);
     d_conv_.download(conv.data(), M_, st_);
@@ -915,6 +1059,7 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
         stats->graph = use_graph;
         stats->digest_waves = tuned_waves_;
         stats->nproblems = P_;
+        stats->sp2_fallbacks = nfallback;
     }
 }
 

@@ -777,10 +777,14 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         if (dE_ < opt.eps && errD < opt.eps && !fp32_phase) { conv = true; break; }
         Eold = Eel;
     }
-    Dfin_.alloc(n2 * nd);
-    PF_HIP(hipMemcpyAsync(Dfin_.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
-    have_fin_ = true;
+    // Only a converged density is kept for the gradient: W = 1/2 D F D gives
+    // dE/dR only at a stationary point.
+    have_fin_ = conv;
     uhf_fin_ = uhf;
+    if (conv) {
+        Dfin_.alloc(n2 * nd);
+        PF_HIP(hipMemcpyAsync(Dfin_.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
+    }
     res.quartets = (double)collect_profile();  // also synchronises the stream
     for (auto &e3 : tev) {
         float a = 0.f, b = 0.f;
@@ -857,7 +861,7 @@ void Engine::setup_grad_pairs() {
 }
 
 void Engine::gradient(double *grad_out, double tau, double *terms) {
-    if (!have_fin_) throw Error("gradient: no converged density (run the SCF first)");
+    if (!have_fin_) throw Error("gradient: no converged density (the SCF did not converge, or was not run)");
     setup_grad_pairs();
     const int n = nbf(), nsh = nshell(), na = mol_.natoms;
     const size_t n2 = (size_t)n * n;

@@ -103,6 +103,11 @@ void pool_release(PooledBlock &b);
 // same thread, ordered after its last use.  dev_free takes the requested size.
 void *dev_alloc(size_t bytes);
 void dev_free(void *p, size_t bytes);
+// Frees the calling thread's cached blocks, the blocks of ended threads and
+// the idle pooled blocks (none of them has pending work).
+void release_device_cache();
+// Bytes held by the calling thread's cache and the ended threads' list.
+size_t cached_device_bytes();
 // HIP events from a per-thread pool: release only events whose recorded work
 // has completed (after a synchronisation or a successful elapsed-time read).
 hipEvent_t event_acquire();

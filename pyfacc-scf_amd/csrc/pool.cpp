@@ -82,15 +82,23 @@ void pool_release(PooledBlock &b) {
 // call can therefore be reused by a later allocation of the same thread:
 // within the call the new owner's work is ordered after the old owner's on
 // the same stream, and across calls the old stream is idle.  Buffers are
-// kept per thread (never handed to another thread) and by exact size class,
-// so a repeated SCF reuses every buffer of the previous one.  At most
-// CACHE_MAX bytes stay cached per thread (beyond: hipFree); blocks over
-// CACHE_BLOCK_MAX (the batched integral store) are never cached.  Cached
-// memory is returned to the driver when the process ends.
+// kept per thread and by exact size class, so a repeated SCF reuses every
+// buffer of the previous one.  At most CACHE_MAX bytes stay cached per
+// thread (beyond: hipFree); blocks over CACHE_BLOCK_MAX (the batched
+// integral store) are never cached.
+//
+// When a thread ends, its blocks move to a process-wide list of idle blocks
+// (every call of that thread has drained its stream, so nothing pending uses
+// them), capped at ORPHAN_MAX bytes, the rest freed; any thread's allocation
+// that misses its own cache takes from that list first.  Threads that come
+// and go therefore reuse the same memory instead of leaking their caches.
+// pf_release_device_cache() frees the calling thread's cache, the idle list
+// and the idle J'_cd pool blocks.
 namespace {
 
 constexpr size_t CACHE_MAX = size_t(8) << 30;
 constexpr size_t CACHE_BLOCK_MAX = size_t(2) << 30;
+constexpr size_t ORPHAN_MAX = size_t(8) << 30;
 
 size_t size_class(size_t bytes) {
     if (bytes <= 4096) return 4096;
@@ -103,14 +111,54 @@ size_t size_class(size_t bytes) {
     return (bytes + mib - 1) / mib * mib;
 }
 
-struct ThreadCache {
-    std::map<std::pair<int, size_t>, std::vector<void *>> free;  // (device, class) -> blocks
+using BlockMap = std::map<std::pair<int, size_t>, std::vector<void *>>;  // (device, class) -> blocks
+
+struct Orphans {
+    std::mutex mu;
+    BlockMap free;
     size_t bytes = 0;
 };
+Orphans &orphans() {
+    static auto *o = new Orphans();  // never destroyed: blocks outlive static teardown
+    return *o;
+}
+
+void free_blocks(BlockMap &m) {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    for (auto &kv : m)
+        for (void *p : kv.second) {
+            (void)hipSetDevice(kv.first.first);
+            (void)hipFree(p);
+        }
+    if (cur >= 0) (void)hipSetDevice(cur);
+    m.clear();
+}
+
+struct ThreadCache {
+    BlockMap free;
+    size_t bytes = 0;
+    ~ThreadCache() {  // thread exit: hand the idle blocks to the process-wide list
+        BlockMap spill;
+        {
+            auto &o = orphans();
+            std::lock_guard<std::mutex> lock(o.mu);
+            for (auto &kv : free)
+                for (void *p : kv.second) {
+                    if (o.bytes + kv.first.second <= ORPHAN_MAX) {
+                        o.free[kv.first].push_back(p);
+                        o.bytes += kv.first.second;
+                    } else {
+                        spill[kv.first].push_back(p);
+                    }
+                }
+        }
+        free_blocks(spill);
+    }
+};
 ThreadCache &thread_cache() {
-    // heap object never destroyed: no hipFree during process teardown
-    static thread_local ThreadCache *c = new ThreadCache();
-    return *c;
+    static thread_local ThreadCache c;
+    return c;
 }
 
 }  // namespace
@@ -127,6 +175,15 @@ void *dev_alloc(size_t bytes) {
             void *p = it->second.back();
             it->second.pop_back();
             tc.bytes -= cls;
+            return p;
+        }
+        auto &o = orphans();
+        std::lock_guard<std::mutex> lock(o.mu);
+        auto jt = o.free.find({dev, cls});
+        if (jt != o.free.end() && !jt->second.empty()) {
+            void *p = jt->second.back();
+            jt->second.pop_back();
+            o.bytes -= cls;
             return p;
         }
     }
@@ -146,6 +203,37 @@ void dev_free(void *p, size_t bytes) {
         return;
     }
     (void)hipFree(p);
+}
+
+void release_device_cache() {
+    auto &tc = thread_cache();
+    free_blocks(tc.free);
+    tc.bytes = 0;
+    {
+        auto &o = orphans();
+        std::lock_guard<std::mutex> lock(o.mu);
+        free_blocks(o.free);
+        o.bytes = 0;
+    }
+    std::vector<PooledBlock> idle;
+    {
+        std::lock_guard<std::mutex> lock(pool_mutex());
+        idle.swap(pool_free());
+    }
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    for (auto &b : idle) {
+        (void)hipSetDevice(b.dev);
+        (void)hipFree(b.p);
+    }
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+
+size_t cached_device_bytes() {
+    size_t n = thread_cache().bytes;
+    auto &o = orphans();
+    std::lock_guard<std::mutex> lock(o.mu);
+    return n + o.bytes;
 }
 
 // ---- per-thread pool of HIP events

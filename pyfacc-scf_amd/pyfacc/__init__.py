@@ -16,6 +16,7 @@ from ._native import (  # noqa: F401
     basis_info,
     batch_scf,
     boys,
+    cached_device_bytes,
     default_options,
     device_count,
     eigh,
@@ -26,6 +27,7 @@ from ._native import (  # noqa: F401
     load_library,
     one_electron,
     projector,
+    release_device_cache,
     scf,
     scf_profile,
 )

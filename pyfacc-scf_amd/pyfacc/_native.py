@@ -23,7 +23,7 @@ class Options(ctypes.Structure):
                 ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int), ("schwarz_inc", ctypes.c_double),
                 ("rebuild_every", ctypes.c_int), ("schwarz_fp32", ctypes.c_double), ("guess", ctypes.c_int),
                 ("level_shift", ctypes.c_double), ("level_shift_until", ctypes.c_double),
-                ("diis_basis", ctypes.c_int), ("mixed_anchor", ctypes.c_double)]
+                ("diis_basis", ctypes.c_int), ("mixed_anchor", ctypes.c_double), ("profile", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -64,6 +64,8 @@ _SIGS = {
     "pf_projector": (_i, [_i, _pd, _d, _pd]),
     "pf_gemm": (_i, [_i, _i, _i, _i, _i, _d, _pd, _pd, _d, _pd]),
     "pf_device_copy": (_i, [ctypes.c_ulonglong, _pd]),
+    "pf_release_device_cache": (_i, []),
+    "pf_cached_device_bytes": (ctypes.c_ulonglong, []),
     "pf_batch_scf": (_i, [_i, _pi32, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _pi32, _pi32,
                           ctypes.c_void_p, _i, _pd, _pi32, _pi32, ctypes.c_void_p]),
 }
@@ -223,7 +225,7 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("t_total", ctypes.c_double), ("t_setup", ctypes.c_double), ("t_eri", ctypes.c_double),
                 ("t_scf", ctypes.c_double), ("eri_values", ctypes.c_double), ("store_bytes", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("graph", ctypes.c_int), ("digest_waves", ctypes.c_int),
-                ("nproblems", ctypes.c_int)]
+                ("nproblems", ctypes.c_int), ("sp2_fallbacks", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -270,6 +272,17 @@ def scf_profile(with_prims=False):
     if with_prims:
         out = out + (np.array(list(pq), dtype=np.int64).reshape(2, 6),)
     return out
+
+
+def release_device_cache():
+    """Free the device blocks the library keeps between calls (this thread's
+    cache, ended threads' idle blocks, idle scratch pools)."""
+    _check(load_library().pf_release_device_cache())
+
+
+def cached_device_bytes():
+    """Bytes of device memory cached for this thread plus ended threads'."""
+    return int(load_library().pf_cached_device_bytes())
 
 
 def device_copy(nbytes):

@@ -44,14 +44,17 @@ def _function():
     class SCFEnergy(torch.autograd.Function):
         @staticmethod
         def forward(ctx, coords, species, nalpha, nbeta, basis, options):
+            # pf_scf_gradient fails (PyfaccError) when the SCF does not converge,
+            # so autograd never receives a gradient of a non-stationary density
             e, g, _ = scf_gradient(species, coords.detach().cpu().double().numpy(), nalpha, nbeta, basis, options)
-            ctx.save_for_backward(torch.as_tensor(g, dtype=coords.dtype, device=coords.device))
-            return coords.new_tensor(e)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=torch.float64, device=coords.device))
+            ctx.in_dtype = coords.dtype
+            return torch.tensor(e, dtype=torch.float64, device=coords.device)
 
         @staticmethod
         def backward(ctx, grad_out):
             (g,) = ctx.saved_tensors
-            return grad_out * g, None, None, None, None, None
+            return (grad_out.to(torch.float64) * g).to(ctx.in_dtype), None, None, None, None, None
 
     return SCFEnergy
 
@@ -60,7 +63,8 @@ _FN = None
 
 
 def scf_energy(coords, species, nalpha, nbeta, basis="sto-3g", options=None, **kw):
-    """Differentiable SCF energy of a (natoms, 3) torch tensor in bohr."""
+    """Differentiable SCF energy (a float64 scalar) of a (natoms, 3) torch
+    tensor in bohr.  Raises if the SCF does not converge."""
     global _FN
     if _FN is None:
         _FN = _function()

@@ -71,6 +71,29 @@ def test_torch_autograd_binding(gpu_lib):
     assert np.abs(x.grad.numpy() - g).max() < 1e-12
 
 
+def test_gradient_refuses_an_unconverged_scf(gpu_lib):
+    """max_iter = 3 stops far from the stationary point: the call fails
+    (status and message) instead of returning dE/dR of a non-stationary
+    density, and the autograd binding raises; the SCF result is still filled."""
+    torch = pytest.importorskip("torch")
+    import pyfacc
+    from pyfacc.grad import scf_energy, scf_gradient
+    sp, xyz = load_xyz("h2o_c1.xyz")
+    with pytest.raises(pyfacc.PyfaccError, match="did not converge"):
+        scf_gradient(sp, xyz, 5, 5, eps=1e-11, max_iter=3)
+    x = torch.tensor(xyz, dtype=torch.float32, requires_grad=True)
+    with pytest.raises(pyfacc.PyfaccError, match="did not converge"):
+        scf_energy(x, sp, 5, 5, eps=1e-11, max_iter=3)
+    # converged: the energy is float64 whatever the coordinates' dtype, the
+    # gradient comes back in the coordinates' dtype
+    e = scf_energy(x, sp, 5, 5, eps=1e-11, max_iter=200)
+    assert e.dtype == torch.float64
+    e.backward()
+    assert x.grad.dtype == torch.float32
+    _, g, _ = scf_gradient(sp, xyz, 5, 5, eps=1e-11, max_iter=200)
+    assert np.abs(x.grad.double().numpy() - g).max() < 1e-6
+
+
 @pytest.mark.parametrize("name,na,nb", [("h2o_c1.xyz", 5, 5), ("oh.xyz", 5, 4)])
 def test_gradient_terms_vs_oracle_finite_differences(gpu_lib, name, na, nb):
     """Each gradient term against oracle finite differences at fixed D and W:

@@ -50,3 +50,28 @@ def test_concurrent_scf_calls_match_sequential(gpu_lib):
     for t, r in enumerate(out):
         assert r["converged"], t
         assert abs(r["energy"] - ref[tasks[t]]) < 1e-8, (t, r["energy"], ref[tasks[t]])
+
+
+def test_short_lived_threads_reuse_cached_device_memory(gpu_lib):
+    """Threads that run one SCF each and end hand their cached device blocks
+    to a shared idle list that later threads reuse: the cache stays the size
+    of one SCF's working set however many threads come and go, and
+    release_device_cache empties it."""
+    import pyfacc
+    sp, xyz = load_xyz("h2o_16.xyz")
+    kw = dict(eps=1e-8, max_iter=150, diis=8, guess=1)
+    e0 = pyfacc.scf(sp, xyz, 80, 80, **kw)["energy"]
+    sizes = []
+    for k in range(6):
+        out = {}
+        th = threading.Thread(target=lambda: out.setdefault("r", pyfacc.scf(sp, xyz, 80, 80, **kw)))
+        th.start()
+        th.join(timeout=100)
+        assert not th.is_alive()
+        assert abs(out["r"]["energy"] - e0) < 1e-8
+        sizes.append(pyfacc.cached_device_bytes())
+    assert sizes[1] > 0
+    assert max(sizes[2:]) <= sizes[1], sizes  # no growth after the first thread's blocks were handed over
+    pyfacc.release_device_cache()
+    assert pyfacc.cached_device_bytes() == 0
+    assert abs(pyfacc.scf(sp, xyz, 80, 80, **kw)["energy"] - e0) < 1e-8  # the library still works after it
