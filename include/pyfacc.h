@@ -83,6 +83,10 @@ typedef struct pf_options {
                              first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient) */
     int profile;          /* 1: time each J/K class kernel with HIP events around it (pf_scf_profile's
                              milliseconds; the events add short gaps to the build), 0: no events */
+    int jk_fallback;      /* testing: direct J/K builds take the paths that large systems need, at any
+                             size: bit 0, exchange rows added with device atomics (rows too long for LDS);
+                             bit 1, J'_cd added with device atomics (ket lists beyond the J'_cd buffer).
+                             0: chosen by size (default) */
 } pf_options;
 
 typedef struct pf_result {
@@ -108,7 +112,7 @@ typedef struct pf_result {
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
  * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error,
- * mixed_anchor 0.1, profile 0. */
+ * mixed_anchor 0.1, profile 0, jk_fallback 0. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */
@@ -146,6 +150,13 @@ int pf_eri_packed(int natoms, const char *species, const int32_t *z, const doubl
 int pf_eri_shell_quartets(int natoms, const char *species, const int32_t *z, const double *coords,
                           const char *basis_name, int renormalize, int nquart, const int32_t *quartets, double *out,
                           int64_t *offsets);
+
+/* Schwarz bounds Q_ab = sqrt(max |(ab|ab)|) over the Cartesian components of
+ * every shell pair, Q[nshell * nshell] symmetric, shells in pf_basis_info's
+ * order; the values the direct J/K builds screen with (Q_ab Q_cd max|D| <
+ * schwarz).  The reference leaves schwarz_matrix empty (schwarz.f90:13-21). */
+int pf_schwarz(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+               int renormalize, double *Q);
 
 /* J[D] (summed over densities) and K[D_k] per density, for ndens = 1 or 2
  * host densities (ndens x nbf x nbf).  eri_mode as in pf_options; precision

@@ -110,6 +110,7 @@ void pf_default_options(pf_options *o) {
     o->diis_basis = PF_DIIS_AO;
     o->mixed_anchor = 0.1;
     o->profile = 0;
+    o->jk_fallback = 0;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }
@@ -183,6 +184,16 @@ int pf_jk(int natoms, const char *species, const int32_t *z, const double *coord
         Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), renormalize != 0,
                    schwarz > 0 ? schwarz : 1e-14);
         eng.jk_host(ndens, D, J, K, eri_mode, precision);
+    });
+}
+
+int pf_schwarz(int natoms, const char *species, const int32_t *z, const double *coords, const char *basis_name,
+               int renormalize, double *Q) {
+    return guarded([&] {
+        if (!Q) throw Error("null output");
+        Input in = make_input(natoms, species, z, coords);
+        Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), renormalize != 0, 1e-14);
+        eng.schwarz_host(Q);
     });
 }
 

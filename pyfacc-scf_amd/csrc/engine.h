@@ -71,6 +71,8 @@ class Engine {
     void jk_host(int ndens, const double *D, double *J, double *K, int mode, int precision);
     void eri_packed_host(double *packed);
     long long schwarz_pairs() const;
+    // Q_ab of every shell pair (nshell^2, symmetric), host output.
+    void schwarz_host(double *Q);
 
     // Full SCF.  D_out (ndens x nbf^2) and eps_out (ndens x nbf) may be null.
     void scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, double *D_out, double *eps_out);
@@ -94,6 +96,7 @@ class Engine {
     std::array<int, 12> prof_launches{};
     std::array<int, 24> prof_class_l{};  // per slot: la lb lc ld as launched (bra | ket)
     bool profile_classes = false;
+    int jk_fallback = 0;  // pf_options::jk_fallback: bit 0 K rows, bit 1 J'_cd through device atomics
 
    private:
     void setup_one_electron();

@@ -197,6 +197,36 @@ void Engine::setup_pairs() {
     pairs_ready_ = true;
 }
 
+void Engine::schwarz_host(double *Q) {
+    const int nsh = nshell();
+    const auto &sh = basis_.shells;
+    std::array<std::vector<std::pair<int, int>>, 3> all;
+    for (int a = 0; a < nsh; ++a)
+        for (int b = 0; b <= a; ++b) {
+            int f = a, s = b;
+            if (sh[b].l > sh[a].l) std::swap(f, s);
+            all[sh[f].l + sh[s].l].push_back({f, s});
+        }
+    const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+    std::fill(Q, Q + (size_t)nsh * nsh, 0.0);
+    for (int c = 0; c < 3; ++c) {
+        PairSet ps;
+        ps.la = cls_l[c][0]; ps.lb = cls_l[c][1];
+        double bound_max = 0.0;
+        fill_pairset(ps, sh, d_shells_.p, all[c], -1.0, bound_max, false, st_);
+        if (ps.n == 0) continue;
+        ps.d_Q.alloc(ps.n);
+        launch_schwarz<double>(ps.la, ps.lb, ps.view64(), d_tab_.p, ps.d_Q.p, st_);
+        std::vector<double> q(ps.n);
+        ps.d_Q.download(q.data(), ps.n, st_);
+        PF_HIP(hipStreamSynchronize(st_));
+        for (int i = 0; i < ps.n; ++i) {
+            Q[(size_t)ps.sa[i] * nsh + ps.sb[i]] = q[i];
+            Q[(size_t)ps.sb[i] * nsh + ps.sa[i]] = q[i];
+        }
+    }
+}
+
 long long Engine::schwarz_pairs() const {
     long long n = 0;
     for (auto &p : pairs_) n += p.n;
@@ -298,7 +328,9 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
         a.count = count_.p + 2 * pbase + classes_[q].slot;
         a.pcount = count_.p + 2 * pbase + 6 + classes_[q].slot;
         a.nket = plan_nket_[q].p; a.item_off = plan_off_[q].p; a.work = plan_work_[q].p;
-        a.joff = plan_joff_[q].p; a.jbuf = static_cast<double *>(jbuf_[q].p); a.jcap = jcap_[q]; a.jhi = plan_jhi_[q].p;
+        a.joff = plan_joff_[q].p; a.jbuf = static_cast<double *>(jbuf_[q].p); a.jhi = plan_jhi_[q].p;
+        a.jcap = (jk_fallback & 2) ? 0 : jcap_[q];  // 0: no ket list fits, J'_cd through device atomics
+        a.katomic = (jk_fallback & 1) ? 1 : 0;
         a.ev0 = profile_classes ? ev[2 * q] : nullptr;
         a.ev1 = profile_classes ? ev[2 * q + 1] : nullptr;
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
@@ -519,6 +551,8 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                     ", nbeta=" + std::to_string(nbeta) + ", nbf=" + std::to_string(n) + ")");
     if (opt.eri_mode == PF_ERI_STORED && opt.precision != PF_PREC_FP64)
         throw Error("stored ERIs are FP64 only");
+    if (opt.jk_fallback < 0 || opt.jk_fallback > 3) throw Error("jk_fallback must be 0..3");
+    jk_fallback = opt.jk_fallback;
 
     setup_one_electron();
     w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); state_.alloc(sp2_state_size(n));

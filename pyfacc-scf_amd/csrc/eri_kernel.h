@@ -620,8 +620,8 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
         PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * W, dyn));
         sh = Shape{dev, a.nbf, std::max(1, per) * std::max(1, ncu), krows, dyn};  // persistent: all that fit
     }
-    a.krows = sh.krows;
-    const size_t dyn = sh.dyn;
+    a.krows = sh.krows && !args.katomic;
+    const size_t dyn = a.krows ? sh.dyn : sizeof(double) * NA * NB;
     const int grid = sh.grid;
     jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
     jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);

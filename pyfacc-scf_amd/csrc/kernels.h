@@ -81,6 +81,7 @@ struct JKArgs {
     int *jhi = nullptr;  // per ket: end of the bra range holding it [ket.n]
     long long jcap = 0;
     int krows = 0;  // direct mode: bra exchange rows summed in LDS
+    int katomic = 0;  // direct mode: force exchange rows to device atomics (the no-LDS-rows path)
     // host side only: when set, recorded on the stream right before and after
     // the class's main ERI kernel (its duration without the helper kernels)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

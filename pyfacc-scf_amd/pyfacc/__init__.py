@@ -30,5 +30,6 @@ from ._native import (  # noqa: F401
     release_device_cache,
     scf,
     scf_profile,
+    schwarz,
 )
 from .helper_functions import compute_spin_electrons, guess_multiplicity, read_xyz, run_scf, symbols_to_z  # noqa: F401

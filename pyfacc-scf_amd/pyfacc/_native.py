@@ -23,7 +23,8 @@ class Options(ctypes.Structure):
                 ("mixed_switch", ctypes.c_double), ("verbose", ctypes.c_int), ("schwarz_inc", ctypes.c_double),
                 ("rebuild_every", ctypes.c_int), ("schwarz_fp32", ctypes.c_double), ("guess", ctypes.c_int),
                 ("level_shift", ctypes.c_double), ("level_shift_until", ctypes.c_double),
-                ("diis_basis", ctypes.c_int), ("mixed_anchor", ctypes.c_double), ("profile", ctypes.c_int)]
+                ("diis_basis", ctypes.c_int), ("mixed_anchor", ctypes.c_double), ("profile", ctypes.c_int),
+                ("jk_fallback", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -54,6 +55,7 @@ _SIGS = {
     "pf_eri_packed": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd]),
     "pf_eri_shell_quartets": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pi32, _pd,
                                    _P(ctypes.c_int64)]),
+    "pf_schwarz": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _pd]),
     "pf_jk": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _pd, _pd, _pd, _i, _i, _d]),
     "pf_scf": (_i, [_i, ctypes.c_char_p, _pi32, _pd, ctypes.c_char_p, _i, _i, _P(Options), _P(Result), _pd, _pd]),
     "pf_scf_profile": (_i, [_P(ctypes.c_ulonglong), _pd, _P(_i), _P(_i), _P(ctypes.c_ulonglong)]),
@@ -156,6 +158,15 @@ def one_electron(species, coords_bohr, basis="sto-3g", renormalize=True):
     S, T, V = (np.zeros((n, n)) for _ in range(3))
     _check(load_library().pf_one_electron(*m.args(), basis.encode(), int(renormalize), _ptr(S), _ptr(T), _ptr(V)))
     return S, T, V
+
+
+def schwarz(species, coords_bohr, basis="sto-3g", renormalize=True):
+    """Schwarz bounds Q_ab = sqrt(max |(ab|ab)|), (nshell, nshell)."""
+    nsh = basis_info(species, coords_bohr, basis, renormalize)["nshell"]
+    m = _Mol(species, coords_bohr)
+    Q = np.zeros((nsh, nsh))
+    _check(load_library().pf_schwarz(*m.args(), basis.encode(), int(renormalize), _ptr(Q)))
+    return Q
 
 
 def eri_packed(species, coords_bohr, basis="sto-3g", renormalize=True):

@@ -1,0 +1,146 @@
+"""Hot-path pieces that only larger runs exercised: the headline (H2O)64 SCF
+in both precisions at the bench settings, screening at the production
+threshold, the Schwarz bounds themselves, the by-reference drop-in entry, UHF
+through run_scf_c, and the two large-system fallback paths of the direct J/K
+build forced on small systems."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_xyz
+
+pytestmark = pytest.mark.gpu
+
+BENCH = dict(eps=1e-8, max_iter=100, diis=8, schwarz=1e-10, guess=1)
+
+
+@pytest.fixture(scope="module")
+def fx():
+    with open(os.path.join(GOLDEN, "oracle_fixtures.json")) as fh:
+        return json.load(fh)
+
+
+def test_h2o64_mixed_and_fp64_at_bench_settings(gpu_lib):
+    """(H2O)64 STO-3G, 448 bf, SAD start, DIIS 8, Schwarz 1e-10, eps 1e-8:
+    both forms converge, |E_mixed - E_FP64| <= 1e-6 Eh (the metric's bar),
+    and each density is an RHF projector: Tr(DS) = 640, DSD = 2D."""
+    import pyfacc
+    sp, xyz = load_xyz("h2o_64.xyz")
+    S, _, _ = pyfacc.one_electron(sp, xyz)
+    out = {}
+    for prec in (0, 1):
+        r = pyfacc.scf(sp, xyz, 320, 320, precision=prec, want_density=True, **BENCH)
+        assert r["converged"], prec
+        D = r["D"]
+        assert abs(np.trace(D @ S) - 640.0) < 1e-8
+        assert np.abs(D @ S @ D - 2.0 * D).max() < 1e-7
+        out[prec] = r
+    dE = out[1]["energy"] - out[0]["energy"]
+    print(f"(H2O)64: E_fp64 = {out[0]['energy']:.10f}, E_mixed - E_fp64 = {dE:.2e} Eh, "
+          f"iterations {out[0]['iterations']} / {out[1]['iterations']}")
+    assert abs(dE) <= 1e-6
+    assert np.abs(out[1]["D"] - out[0]["D"]).max() < 1e-5
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_h2o16_production_screening_energy_bound(gpu_lib, fx, prec):
+    """(H2O)16 STO-3G at the default screening (Q_ab Q_cd max|D| < 1e-10, and
+    1e-12 on FP64 increments) against the oracle's energy from all integrals:
+    the screened energy stays within 1e-8 Eh (measured on MI355X: 4.4e-9 Eh
+    in FP64; the screened quartets' contributions, each below 1e-10, add up)."""
+    import pyfacc
+    e = fx["h2o16_sto3g"]
+    sp, xyz = load_xyz("h2o_16.xyz")
+    r = pyfacc.scf(sp, xyz, 80, 80, precision=prec, eps=1e-10, max_iter=200, diis=8, schwarz=1e-10, guess=1)
+    assert r["converged"]
+    print(f"(H2O)16 precision {prec}: E - E_oracle = {r['energy'] - e['energy']:.2e} Eh")
+    assert abs(r["energy"] - e["energy"]) < 1e-8
+
+
+@pytest.mark.parametrize("name,basis,nw", [("h2o_c1.xyz", "sto-3g", 0), ("oh.xyz", "sto-6g", 0),
+                                           ("h2o_16.xyz", "sto-3g", 4), ("h2o_16.xyz", "sto-6g", 2)])
+def test_schwarz_bounds_vs_oracle(gpu_lib, name, basis, nw):
+    """Q_ab of every shell pair (the screening bound of the direct builds)
+    against the oracle's, from its McMurchie-Davidson (ab|ab) blocks."""
+    import oracle
+    import pyfacc
+    sp, xyz = load_xyz(name)
+    if nw:
+        sp, xyz = sp[:3 * nw], xyz[:3 * nw]
+    Q = pyfacc.schwarz(sp, xyz, basis)
+    Qo = oracle.Molecule(sp, xyz, basis).schwarz()
+    assert Q.shape == Qo.shape
+    assert np.abs(Q - Q.T).max() == 0.0
+    assert np.abs(Q - Qo).max() <= 1e-12 * max(1.0, np.abs(Qo).max())
+
+
+def test_run_scf_interface_py_by_reference(gpu_lib):
+    """The by-reference entry (main.f90:15, scf_interface.c:7-9) as
+    single_file.py declares it: every scalar by pointer, the basis name of
+    length *len_basis_set with no NUL after it."""
+    import oracle
+    import pyfacc
+    lib = ctypes.CDLL(pyfacc.LIB_PATH)
+    i32, P = ctypes.c_int32, ctypes.POINTER
+    lib.run_scf_interface_py.argtypes = [P(i32), P(i32), P(i32), P(i32), P(i32), ctypes.c_char_p,
+                                         P(ctypes.c_double), P(ctypes.c_double), P(i32), P(ctypes.c_double),
+                                         ctypes.c_char_p, P(i32), P(i32)]
+    lib.run_scf_interface_py.restype = None
+    for name, na, nb in (("h2o_c1.xyz", 5, 5), ("oh.xyz", 5, 4)):
+        sp, xyz = load_xyz(name)
+        n = len(sp)
+        z = (i32 * n)(*pyfacc.symbols_to_z(sp))
+        coords = (ctypes.c_double * (3 * n))(*xyz.reshape(-1))
+        species = b"".join(s.ljust(2).encode() for s in sp)
+        # the basis name sits in a longer buffer: only its first 6 bytes count
+        basis = ctypes.create_string_buffer(b"sto-3gXYZ", 9)
+        out = ctypes.c_double(0.0)
+        lib.run_scf_interface_py(i32(n), i32(na + nb), i32(0), i32(na), i32(nb), species, coords,
+                                 ctypes.c_double(1e-10), i32(300), ctypes.byref(out), basis, i32(6), z)
+        ref = oracle.Molecule(sp, xyz).scf(na, nb, eps=1e-10, max_iter=300)
+        assert ref["converged"]
+        assert abs(out.value - ref["energy"]) < 1e-9, (name, out.value, ref["energy"])
+
+
+def test_run_scf_c_uhf_doublet_vs_oracle(gpu_lib):
+    """run_scf_c with nalpha != nbeta dispatches to UHF (the intent of
+    main.f90:57-61): the OH radical against the oracle's UHF, plain Roothaan."""
+    import oracle
+    import pyfacc
+    lib = ctypes.CDLL(pyfacc.LIB_PATH)
+    sp, xyz = load_xyz("oh.xyz")
+    z = (ctypes.c_int32 * 2)(8, 1)
+    coords = (ctypes.c_double * 6)(*xyz.reshape(-1))
+    out = ctypes.c_double(0.0)
+    lib.run_scf_c(2, 9, 0, 5, 4, b"O H ", coords, ctypes.c_double(1e-10), 300, ctypes.byref(out), b"sto-3g", z)
+    ref = oracle.Molecule(sp, xyz).scf(5, 4, eps=1e-10, max_iter=300)
+    assert ref["converged"]
+    assert abs(out.value - ref["energy"]) < 1e-9
+
+
+@pytest.mark.parametrize("fallback", [1, 2, 3])
+@pytest.mark.parametrize("case", ["rhf", "uhf"])
+def test_forced_large_system_jk_paths_vs_oracle(gpu_lib, fallback, case):
+    """The direct build's paths for systems too large for its LDS exchange
+    rows (bit 0: K through device atomics) or its J'_cd buffer (bit 1: J'_cd
+    through device atomics), forced on (H2O)4 / (H2O)3OH: FP64 energies at
+    1e-9 Eh of the oracle, mixed within 1e-8 of FP64."""
+    import oracle
+    import pyfacc
+    sp, xyz = load_xyz("h2o_16.xyz")
+    sp, xyz = sp[:12], xyz[:12]
+    if case == "uhf":  # the last H dropped: (H2O)3OH doublet, 39 electrons
+        sp, xyz = sp[:11], xyz[:11]
+    ne = int(sum(pyfacc.symbols_to_z(sp)))
+    na, nb = (ne + 1) // 2, ne // 2
+    ref = oracle.Molecule(sp, xyz).scf(na, nb, eps=1e-10, max_iter=300, diis=8)
+    assert ref["converged"]
+    kw = dict(eps=1e-10, max_iter=300, diis=8, schwarz=1e-14, schwarz_inc=1e-14, jk_fallback=fallback)
+    r64 = pyfacc.scf(sp, xyz, na, nb, precision=0, **kw)
+    rmx = pyfacc.scf(sp, xyz, na, nb, precision=1, **kw)
+    assert r64["converged"] and rmx["converged"]
+    assert abs(r64["energy"] - ref["energy"]) < 1e-9
+    assert abs(rmx["energy"] - r64["energy"]) < 1e-8
