@@ -246,6 +246,9 @@ def main():
                     help="instances of the batched water-cluster workload (0: skip)")
     ap.add_argument("--no-reference-inputs", action="store_true",
                     help="skip the SCFs of the reference's glycine-chain and RNA inputs")
+    ap.add_argument("--jk-only", action="store_true",
+                    help="only one FP64 and one FP32 full direct J/K build of the workload at a seeded RHF "
+                         "density (after one untimed build each): the short command for rocprofv3 counter runs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the byte-counter "
                          "calibration kernel for rocprofv3 --pmc runs)")
@@ -266,6 +269,19 @@ def main():
     species, xyz = load_xyz(args.xyz)
     nelec = sum(pyfacc.helper_functions.periodic_table[s] for s in species)
     nocc = nelec // 2
+    if args.jk_only:
+        nbf = pyfacc.basis_info(species, xyz)["nbf"]
+        rng = np.random.default_rng(7)
+        C = np.linalg.qr(rng.standard_normal((nbf, nbf)))[0][:, :nocc]
+        D = 2.0 * C @ C.T
+        out = {}
+        for fp32 in (False, True):
+            pyfacc.jk(species, xyz, D, schwarz=1e-10, fp32=fp32)
+            t0 = time.perf_counter()
+            J, K = pyfacc.jk(species, xyz, D, schwarz=1e-10, fp32=fp32)
+            out["fp32" if fp32 else "fp64"] = {"s": time.perf_counter() - t0, "J_fro": float(np.linalg.norm(J))}
+        print(json.dumps({"jk_only": out, "nbf": nbf}))
+        return
     # superposition-of-atomic-densities start for both forms (the core guess of
     # the reference is timed beside them below)
     common = dict(eps=1e-8, max_iter=100, diis=8, schwarz=1e-10, guess=1)
@@ -308,6 +324,15 @@ def main():
 
     e64 = fp64[-1]["energy"]
     emx = mixed[-1]["energy"]
+    # the CPU reference energy of the headline cluster: the oracle's energy
+    # functional at a stationary density of its own SCF equations (one oracle
+    # Fock build; tests/golden/make_h2o64_certificate.py)
+    e_cpu = None
+    cert = os.path.join(ROOT, "tests", "golden", "h2o64_certificate.json")
+    if os.path.abspath(args.xyz) == os.path.join(ROOT, "pyfacc-scf_amd", "data", "molecules", "h2o_64.xyz") and \
+            os.path.exists(cert):
+        with open(cert) as fh:
+            e_cpu = json.load(fh)["energy_functional"]
     # ---- roofline of the dominant kernel of the timed region (mixed SCF)
     flat_ms = ms_mx.reshape(-1)
     k = int(np.argmax(flat_ms))
@@ -362,6 +387,8 @@ def main():
                        "dE_mixed_vs_fp64": core["mixed"][1]["energy"] - core["fp64"][1]["energy"]},
         "speedup_mixed_vs_fp64": (t_fp64 / args.steps) / (t_mixed / args.steps),
         "dE_mixed_vs_fp64": emx - e64,
+        "dE_mixed_vs_cpu_reference": None if e_cpu is None else emx - e_cpu,
+        "dE_fp64_vs_cpu_reference": None if e_cpu is None else e64 - e_cpu,
         "energy_fp64": e64,
         "iterations_fp64": fp64[-1]["iterations"],
         "iterations_mixed": mixed[-1]["iterations"],

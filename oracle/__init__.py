@@ -69,7 +69,9 @@ def parse_g94(path, species):
     basis.f90: element blocks in file order; within a block each shell line
     creates one shell per atom of that element (atom order); an SP line
     creates the S shells of all those atoms, then their P shells
-    (basis.f90:133-190).  Fortran ``D`` exponents are accepted.
+    (basis.f90:133-190).  Fortran ``D`` exponents are accepted.  ``D`` shell
+    lines (which the reference's parser passes over, basis.f90:76-82) give
+    Cartesian d shells.
     """
     with open(path) as fh:
         lines = fh.read().splitlines()
@@ -104,6 +106,8 @@ def parse_g94(path, species):
                 shells += [dict(l=0, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
             elif kind == "P":
                 shells += [dict(l=1, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
+            elif kind == "D":  # Cartesian d (6 components); basis.f90:76-82 has no branch for it
+                shells += [dict(l=2, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
             elif kind == "SP":
                 shells += [dict(l=0, atom=a, exps=exps, coefs=[r[1] for r in rows]) for a in atoms]
                 shells += [dict(l=1, atom=a, exps=exps, coefs=[r[2] for r in rows]) for a in atoms]

@@ -617,7 +617,6 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     double Eold = 0.0, Eel = 0.0, last_dD = 1e300;
     int it = 0, since_full = 0;
     bool anchored = false;
-    DevBuf<double> Ganchor, Danchor;  // G and D of the anchor's full FP64 build
     bool conv = false;
     // per iteration: events at the start of the Fock build, its end, and the end
     // of the density/energy step (t_fock and t_diag without host round trips)
@@ -660,18 +659,10 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         const bool anchor_now = fp32_phase && !switch_now && !anchored && last_dD < opt.mixed_anchor;
         const bool rebuild = !fp32_phase && incremental && opt.rebuild_every > 0 &&
                              since_full >= opt.rebuild_every;
-        if (switch_now && anchored) {
-            // The switch after an anchor: G = G_anchor + G_FP64[D - D_anchor], all in
-            // FP64 (the anchor's G was a full FP64 build), screened like the FP64
-            // incremental builds -- the FP32 increments since the anchor drop out.
-            fp32_phase = false;
-            PF_HIP(hipMemcpyAsync(G.p, Ganchor.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
-            dev_axpby(n2 * nd, 1.0, D.p, -1.0, Danchor.p, dD.p, st_);
-            build_g<double>(dD.p, uhf ? dD.p + n2 : nullptr, G0, G1, true, gmode, opt.schwarz_inc);
-            ++res.fock_builds_fp64;
-            since_full = 0;
-        } else if (switch_now || anchor_now || rebuild || !incremental) {
-            // full FP64 build: the anchor of the FP32 phase, or the switch to FP64
+        if (switch_now || anchor_now || rebuild || !incremental) {
+            // full FP64 build: the anchor of the FP32 phase, or the switch to FP64 (the
+            // last full build of a mixed SCF: every FP32 increment drops out of G, and
+            // only FP64 increments follow it)
             DevBuf<double> Gs;
             // DIIS keeps its subspace: at the anchor the error vectors (~|dD|) dwarf
             // the offset, and those entries leave the subspace before it matters;
@@ -694,12 +685,6 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
             build_g<double>(Dcur0, Dcur1, G0, G1, false, gmode);
             ++res.fock_builds_fp64;
             since_full = 0;
-            if (anchor_now) {  // kept for the switch
-                Ganchor.alloc(n2 * nd);
-                Danchor.alloc(n2 * nd);
-                PF_HIP(hipMemcpyAsync(Ganchor.p, G.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
-                PF_HIP(hipMemcpyAsync(Danchor.p, D.p, sizeof(double) * n2 * nd, hipMemcpyDeviceToDevice, st_));
-            }
             if (Gs.p) {
                 // The stored DIIS Fock matrices share (nearly) this offset
                 // d = G_fp32 - G_fp64, accumulated mostly by the first builds:

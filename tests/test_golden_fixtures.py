@@ -75,3 +75,19 @@ def test_scf_energy_fixtures(fx, name):
     r = _mol(e).scf(e["nalpha"], e["nbeta"], eps=1e-10, max_iter=200, diis=8)
     assert r["converged"]
     assert abs(r["energy"] - e["energy"]) < 1e-10
+
+
+def test_h2o64_oracle_certificate_is_a_stationary_rhf_density():
+    """The committed (H2O)64 certificate (make_h2o64_certificate.py): the
+    density it was made from satisfies the oracle's SCF equations (residual
+    max |F D S - S D F| from one oracle Fock build), holds 640 electrons and
+    is idempotent; its energy functional is then the oracle's SCF energy of the
+    headline cluster up to O(residual^2)."""
+    import json
+    with open(os.path.join(GOLDEN, "h2o64_certificate.json")) as fh:
+        c = json.load(fh)
+    assert c["oracle_schwarz"] <= 1e-12
+    assert c["stationarity_residual_max"] < 1e-7
+    assert abs(c["trace_DS"] - 640.0) < 1e-9
+    assert c["idempotency_max"] < 1e-10
+    assert -4797.5 < c["energy_functional"] < -4797.3

@@ -112,8 +112,15 @@ def _deriv(fun, centres, dirs):
             cc[ci][di] = v
         return fun(cc)
 
+    # a coordinate named k times is one variable differentiated k times
+    uniq = []
+    for v in vars_:
+        if v not in uniq:
+            uniq.append(v)
+    vars_ = uniq
     x0 = [base[ci][di] for ci, di in vars_]
-    return mp.diff(f, x0, tuple(1 for _ in vars_))
+    orders = tuple(sum(1 for w in dirs if tuple(w) == v) for v in vars_)
+    return mp.diff(f, x0, orders)
 
 
 # p_dir on centre 0 (exponent a): (1/2a) d/dA_dir
@@ -150,6 +157,49 @@ def test_p_eri_from_derivatives(ls, dirs):
             idx.append(0)
     ref = float(scale * _deriv(lambda cc: ss_eri(a, b, c, d, *cc), [A, B, C, Dc], var))
     assert e[tuple(idx)] == pytest.approx(ref, rel=1e-9, abs=1e-14)
+
+
+# d components from second derivatives of the s closed forms:
+# (x - A_x)^2 g = (1/4a^2) d^2g/dA_x^2 + (1/2a) g, (x - A_x)(y - A_y) g = (1/4a^2) d^2g/dA_x dA_y
+D_COMPS = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]  # xx xy xz yy yz zz
+
+
+def _d_from_s(fun, centres, ci, e, comp, extra=()):
+    i, j = D_COMPS[comp]
+    v = _deriv(fun, centres, [(ci, i), (ci, j)] + list(extra)) / (4 * e * e)
+    if i == j:
+        if extra:
+            v += _deriv(fun, centres, list(extra)) / (2 * e)
+        else:
+            v += fun([list(map(mp.mpf, x)) for x in centres]) / (2 * e)
+    return v
+
+
+@pytest.mark.parametrize("comp", range(6))
+def test_d_one_electron_from_derivatives(comp):
+    S, T, V = oracle.prim_1e(2, 0, a, b, A, B, Z=[2], C=[C])
+    for mat, fun in ((S, lambda cc: ss_overlap(a, b, cc[0], cc[1])),
+                     (T, lambda cc: ss_kinetic(a, b, cc[0], cc[1])),
+                     (V, lambda cc: ss_nuclear(a, b, cc[0], cc[1], C, Z=2))):
+        ref = float(_d_from_s(fun, [A, B], 0, a, comp))
+        assert mat[comp, 0] == pytest.approx(ref, rel=1e-9, abs=1e-13)
+
+
+@pytest.mark.parametrize("comp,pc", [(0, 0), (1, 2), (3, 1), (4, 0), (5, 2)])
+def test_d_p_eri_from_derivatives(comp, pc):
+    """(d s | p s): a d component on centre A, p on centre C."""
+    e = oracle.prim_eri(2, 0, 1, 0, a, b, c, d, A, B, C, Dc)
+    fun = lambda cc: ss_eri(a, b, c, d, *cc)
+    ref = _d_from_s(fun, [A, B, C, Dc], 0, a, comp, extra=[(2, pc)]) / (2 * c)
+    assert e[comp, 0, pc, 0] == pytest.approx(float(ref), rel=1e-8, abs=1e-13)
+
+
+def test_parse_d_shells():
+    """STO-3G bromine: the D line gives one l = 2 shell (3d), in file order."""
+    m = oracle.Molecule(["Br", "H"], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 2.67]]))
+    ls = [s["l"] for s in m.shells]
+    # element blocks in file order (H before Br), then Br's S, SP, SP, SP, D
+    assert ls == [0, 0, 0, 1, 0, 1, 0, 1, 2] and m.nbf == 1 + 4 + 3 * 3 + 6
 
 
 # ---------------------------------------------------------------- contracted
