@@ -229,6 +229,23 @@ int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int
                  const char *basis_name, const int32_t *nalpha, const int32_t *nbeta, const pf_options *opt,
                  int use_graph, double *energy, int32_t *iterations, int32_t *converged, pf_batch_stats *stats);
 
+/* The projector of pf_projector for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc).
+ * No counterpart in the reference.  F (n x n, symmetric, row-major) and P (the
+ * projector onto the nocc lowest eigenvectors of F) are caller-owned DEVICE
+ * buffers; stream is a hipStream_t (NULL: the null stream).  The call only
+ * enqueues work: exactly max_steps (1..200) purification steps (steps after
+ * convergence do nothing; 100 covers any gap of practical interest), no host
+ * wait, no hipMalloc/hipFree, no memset.  Its working memory (2 n^2 + O(n)
+ * doubles) is a fixed device array of the library, one per GPU, sized for
+ * n <= PF_PROJECTOR_NOALLOC_NMAX (about 16 MiB), held until the process ends;
+ * larger n is rejected (PF_ERR_INPUT).  Calls that overlap on one GPU (two
+ * streams, two threads) share that array and race: serialise them.  The first
+ * call on each GPU resolves the array's address (no allocation); make it
+ * before a graph capture. */
+#define PF_PROJECTOR_NOALLOC_NMAX 1024
+int pf_projector_noalloc(int n, const double *F, double nocc, double *P, int max_steps, void *stream);
+
 /* Device memory kept between calls.  Every call drains its own stream, so the
  * library caches released device blocks per calling thread for its next call
  * (up to 8 GiB per thread); a thread that ends hands its blocks to a

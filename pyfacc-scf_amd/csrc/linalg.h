@@ -64,5 +64,9 @@ bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_
 size_t sp2_state_size(int n);
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
                 int max_iter = 100, int first_check = 8);
+// The same projector into P with exactly max_steps step launches, no host
+// round trip and no allocation (X, work: n x n; state: sp2_state_size(n)).
+void sp2_project_fixed(int n, const double *Fp, double nocc, double *P, double *X, double *work, double *state,
+                       int max_steps, hipStream_t st);
 
 }  // namespace pf

@@ -804,4 +804,28 @@ int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, d
     return used;
 }
 
+// The same purification with no host round trip and no allocation: every
+// buffer comes from the caller (X and work n x n, state sp2_state_size(n)
+// doubles), exactly max_steps step kernels are launched (steps after
+// convergence only copy or do nothing), and the symmetrised projector goes to
+// P.  Nothing here synchronises, so the sequence can be captured in a graph.
+void sp2_project_fixed(int n, const double *Fp, double nocc, double *P, double *X, double *work, double *state,
+                       int max_steps, hipStream_t st) {
+    const int nt = (n + 31) / 32;
+    const dim3 tiles(nt * (nt + 1) / 2);
+    double *rows = state + 8 + 6 * (size_t)sp2_tiles(n);
+    gershgorin_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, Fp, rows, rows + n);
+    gershgorin_bounds_kernel<<<1, 1024, 0, st>>>(n, rows, rows + n, state);
+    sp2_init_kernel<<<tiles, 256, 0, st>>>(n, Fp, state, X);
+    const double tol = 1e-13 * std::max(1.0, nocc);
+    double *cur = X, *nxt = work;
+    for (int it = 0; it < max_steps; ++it) {
+        sp2_step_kernel<<<tiles, 64 * SW, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
+        std::swap(cur, nxt);
+    }
+    const dim3 g2((n + 127) / 128, n);
+    symmetrize_kernel<<<g2, 128, 0, st>>>(n, cur, P);
+    PF_HIP(hipGetLastError());
+}
+
 }  // namespace pf
