@@ -56,44 +56,48 @@ struct BoysTraits<float> {
     __device__ static float rsqrt_pi_over(float x) { return sqrtf(3.14159265358979f / x); }
 };
 
-// 1/sqrt(x) for x > 0 in the normal range: hardware estimate, then (FP64) two
-// Newton steps y <- y (3/2 - x y^2 / 2), each squaring the relative error.
+// 1/sqrt(x) for x > 0 in the normal range: hardware estimate (measured on
+// MI355X: relative error <= 5.2e-8 over 1e-3 .. 1e4), then one Newton step
+// y <- y (3/2 - x y^2 / 2), which squares it: <= 4.1e-15 measured, so a
+// primitive quartet's prefactor and exponent carry ~1e-14 relative error,
+// far under the 1e-12 absolute accuracy of the contracted integrals.
 __device__ __forceinline__ double rsqrt_nr(double x) {
     double y = __builtin_amdgcn_rsq(x);
     const double h = 0.5 * x;
-    y = fma(y, fma(-h, y * y, 0.5), y);
     y = fma(y, fma(-h, y * y, 0.5), y);
     return y;
 }
 __device__ __forceinline__ float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Workgroup copy of the Boys table for one top order M (ERI kernels).
-// Grid point k holds the Taylor coefficients of F_M about x_k, pre-divided
-// by j!, c_j = F_{M+j}(x_k) / j! (j < terms), and, for M > 0, exp(-x_k), so
-// that the evaluation needs neither factorials nor an exp call:
-//   F_M(x)    = sum_j c_j d^j,  d = x_k - x, |d| <= h/2
-//   exp(-x)   = exp(-x_k) exp(d), exp(d) by its Taylor series to d^8 in
-//               FP64 ((1/32)^9 / 9! < 1e-19), to d^5 in FP32
+// Grid point k holds NC Taylor coefficients about x_k, pre-divided by j!:
+//   M = 0:  c_j = F_0(x_k) ... as F_j(x_k) / j!, j < NT      F_0(x) = sum_j c_j d^j
+//   M >= 1: c_j = F_{M-1+j}(x_k) / j!, j < NT + 1          F_{M-1}(x) = sum_j c_j d^j
+// with d = x_k - x, |d| <= h/2 (dF_m/dx = -F_{m+1}).  For M >= 1 one Horner
+// pass gives the polynomial p = F_{M-1} and its derivative p' = dp/dd =
+// -dF_{M-1}/dx = F_M together; then exp(-x) follows exactly from the
+// recursion identity, e = (2M - 1) F_{M-1} - 2x F_M, and the lower orders from
+// the downward recursion F_m = (2x F_{m+1} + e) / (2m + 1): no exp call, no
+// exp series and no divide.  The one extra coefficient keeps F_M (the
+// derivative of a degree-NT polynomial) as accurate as F_{M-1}:
+// the truncation of p' is about (NT+1) c_{NT+1} (h/2)^NT, below 1e-15 in FP64.
 template <int M, typename R>
 struct BoysLds {
     static constexpr int NT = BoysTraits<R>::terms;
-    static constexpr int STRIDE = NT + (M > 0 ? 1 : 0);
+    static constexpr int NC = M > 0 ? NT + 1 : NT;  // coefficients per grid point
+    static constexpr int STRIDE = NC;
     static constexpr int SIZE = BOYS_NGRID * STRIDE;
+    static constexpr int BASE = M > 0 ? M - 1 : 0;  // order of the stored polynomial
+    static_assert(BASE + NC - 1 <= BOYS_MMAX, "Boys table too short for this order");
 
     // Fill lds[SIZE] from the full table (stride BOYS_MMAX+1); all threads of
     // the workgroup take part, the caller synchronises.
     __device__ static void fill(R *lds, const R *__restrict__ tab, int tid, int nthreads) {
         for (int k = tid; k < SIZE; k += nthreads) {
             const int g = k / STRIDE, j = k - g * STRIDE;
-            R v;
-            if (j < NT) {
-                R f = R(1);
-                for (int t = 2; t <= j; ++t) f *= R(t);
-                v = tab[g * (BOYS_MMAX + 1) + M + j] / f;
-            } else {
-                v = R(exp(-double(g) * BOYS_H));
-            }
-            lds[k] = v;
+            R f = R(1);
+            for (int t = 2; t <= j; ++t) f *= R(t);
+            lds[k] = tab[g * (BOYS_MMAX + 1) + BASE + j] / f;
         }
     }
 
@@ -102,28 +106,28 @@ struct BoysLds {
             const int k = int(x * R(1.0 / BOYS_H) + R(0.5));
             const R d = R(k) * R(BOYS_H) - x;
             const R *t = lds + k * STRIDE;
-            R acc = t[NT - 1];
+            if constexpr (M == 0) {
+                R acc = t[NT - 1];
 #pragma unroll
-            for (int j = NT - 2; j >= 0; --j) acc = fma(acc, d, t[j]);
-            F[M] = acc;
-            if constexpr (M > 0) {
-                R ed;
-                if constexpr (std::is_same<R, double>::value) {
-                    ed = fma(R(1.0 / 40320.0), d, R(1.0 / 5040.0));
-                    ed = fma(ed, d, R(1.0 / 720.0));
-                    ed = fma(ed, d, R(1.0 / 120.0));
-                } else {
-                    ed = R(1.0 / 120.0);
+                for (int j = NT - 2; j >= 0; --j) acc = fma(acc, d, t[j]);
+                F[0] = acc;
+            } else {
+                // p = sum_j t_j d^j and p' = dp/dd by one Horner pass
+                R p = fma(t[NC - 1], d, t[NC - 2]);
+                R dp = t[NC - 1];
+#pragma unroll
+                for (int j = NC - 3; j >= 0; --j) {
+                    dp = fma(dp, d, p);
+                    p = fma(p, d, t[j]);
                 }
-                ed = fma(ed, d, R(1.0 / 24.0));
-                ed = fma(ed, d, R(1.0 / 6.0));
-                ed = fma(ed, d, R(0.5));
-                ed = fma(ed, d, R(1));
-                ed = fma(ed, d, R(1));
-                const R e = t[NT] * ed;
-                const R x2 = x + x;
+                F[M - 1] = p;
+                F[M] = dp;
+                if constexpr (M > 1) {
+                    const R x2 = x + x;
+                    const R e = fma(-x2, dp, R(2 * M - 1) * p);  // exp(-x)
 #pragma unroll
-                for (int m = M - 1; m >= 0; --m) F[m] = fma(x2, F[m + 1], e) * R(1.0 / (2 * m + 1));
+                    for (int m = M - 2; m >= 0; --m) F[m] = fma(x2, F[m + 1], e) * R(1.0 / (2 * m + 1));
+                }
             }
         } else {
             // F_0 = sqrt(pi/x)/2; upward F_{m+1} = (2m+1)/(2x) F_m.  The
