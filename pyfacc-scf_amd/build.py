@@ -23,7 +23,7 @@ FLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
 ERI_CLASSES = ["ssss", "psss", "ssps", "psps", "ppss", "sspp", "ppps", "pspp", "pppp"]
-SOURCES = ["eri_%s.hip" % c for c in ERI_CLASSES] + ["grad_%s.hip" % c for c in ERI_CLASSES] + ["grad.hip", "eri.hip", "onee.hip", "linalg.hip", "engine.hip", "batch.hip", "eri_batch.hip", "pairs.hip", "eri_export.hip", "testkern.hip", "noalloc.hip", "basis.cpp", "pool.cpp", "abi.cpp"]
+SOURCES = ["eri_%s.hip" % c for c in ERI_CLASSES] + ["grad_%s.hip" % c for c in ERI_CLASSES] + ["grad.hip", "eri.hip", "onee.hip", "linalg.hip", "engine.hip", "batch.hip", "eri_batch.hip", "pairs.hip", "eri_export.hip", "generic.hip", "testkern.hip", "noalloc.hip", "basis.cpp", "pool.cpp", "abi.cpp"]
 
 
 def _compile(src):

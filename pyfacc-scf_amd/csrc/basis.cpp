@@ -194,9 +194,10 @@ Basis build_basis_blocks(const BasisBlocks &blocks, const std::vector<std::strin
             if (sh.kind == "S") add(0, 1);
             else if (sh.kind == "P") add(1, 1);
             else if (sh.kind == "SP") { add(0, 1); add(1, 2); }
+            else if (sh.kind == "D") add(2, 1);  // basis.f90:76-82 has no D branch; the Cartesian code is general
             else
                 throw Error("basis file: shell type '" + sh.kind + "' of " + blk.label +
-                            " is not supported (s and p shells only)");
+                            " is not supported (s, p and d shells only)");
         }
     }
     // every atom must have received functions

@@ -577,6 +577,8 @@ void BatchEngine::setup() {
     for (int m = 0; m < M_; ++m) {
         const auto &bm = mols_[m];
         Basis bs = build_basis_blocks(blocks, bm.species, bm.mol, renorm_);
+        for (const auto &sh : bs.shells)
+            if (sh.l >= 2) throw Error("batch: molecule " + std::to_string(m) + ": s and p shells only (d shells: pf_scf)");
         for (auto &s : bs.shells) shells_.push_back(s);
         shoff[m + 1] = (int)shells_.size();
         aoff[m + 1] = aoff[m] + bm.mol.natoms;

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "generic.h"
 #include "kernels.h"
 #include "pf_common.h"
 
@@ -136,6 +137,14 @@ class Engine {
     std::vector<QuartetClass> classes_;
     bool pairs_ready_ = false;
     bool one_e_ready_ = false;
+    // Shell pairs with a d shell (generic McMurchie-Davidson path): the kept
+    // ones (Q * Qmax >= schwarz), first shell the higher l, sorted by Q, and
+    // the ket list of their quartets (these pairs first, then every kept s/p pair).
+    bool has_d_ = false;
+    int ngen_ = 0, nket_gen_ = 0;
+    DevBuf<int> d_gsa_, d_gsb_, d_ksa_, d_ksb_;
+    DevBuf<double> d_gQ_, d_kQ_;
+    GenQuartets gen_quartets() const;
     // J/K accumulators
     DevBuf<double> J64_, K64a_, K64b_, Dsh64_, dmax64_;
     DevBuf<float> Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;

@@ -63,6 +63,7 @@ Engine::Engine(const Molecule &mol, const std::vector<std::string> &species, con
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error("no GPU device visible");
     basis_ = build_basis(basis_path, species, mol_, renormalise);
+    for (const auto &sh : basis_.shells) has_d_ = has_d_ || sh.l >= 2;
     PF_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     const int nsh = nshell();
     d_shells_.upload(basis_.shells.data(), nsh, st_);
@@ -94,6 +95,19 @@ void Engine::setup_one_electron() {
     const size_t n2 = (size_t)nbf() * nbf();
     S_.alloc(n2); T_.alloc(n2); V_.alloc(n2); H_.alloc(n2);
     launch_onee(d_shells_.p, nshell(), nbf(), mol_.natoms, d_axyz_.p, d_Z_.p, d_tab_.p, S_.p, T_.p, V_.p, st_);
+    if (has_d_) {  // pairs with a d shell: generic kernel
+        std::vector<int> pa, pb;
+        const auto &sh = basis_.shells;
+        for (int a = 0; a < nshell(); ++a)
+            for (int b = 0; b <= a; ++b)
+                if (sh[a].l >= 2 || sh[b].l >= 2) { pa.push_back(a); pb.push_back(b); }
+        DevBuf<int> dpa, dpb;
+        dpa.upload(pa.data(), pa.size(), st_);
+        dpb.upload(pb.data(), pb.size(), st_);
+        launch_onee_generic(d_shells_.p, dpa.p, dpb.p, (int)pa.size(), nbf(), mol_.natoms, d_axyz_.p, d_Z_.p,
+                            d_tab_.p, S_.p, T_.p, V_.p, st_);
+        PF_HIP(hipStreamSynchronize(st_));  // dpa, dpb go back to the block cache
+    }
     dev_axpby(n2, 1.0, T_.p, 1.0, V_.p, H_.p, st_);
     one_e_ready_ = true;
 }
@@ -114,16 +128,30 @@ void Engine::setup_pairs() {
     const int nsh = nshell();
     const auto &sh = basis_.shells;
     std::array<std::vector<std::pair<int, int>>, 3> all;
+    std::vector<std::pair<int, int>> gall;  // pairs with a d shell
     for (int a = 0; a < nsh; ++a)
         for (int b = 0; b <= a; ++b) {
             int f = a, s = b;
             if (sh[b].l > sh[a].l) std::swap(f, s);
-            all[sh[f].l + sh[s].l].push_back({f, s});
+            if (sh[f].l >= 2) gall.push_back({f, s});
+            else all[sh[f].l + sh[s].l].push_back({f, s});
         }
     double bound_max = 0.0;
     auto fill = [&](PairSet &ps, const std::vector<std::pair<int, int>> &list, double kref, bool want32) {
         fill_pairset(ps, sh, d_shells_.p, list, kref, bound_max, want32, st_);
     };
+    std::vector<double> gQall(gall.size());
+    if (!gall.empty()) {
+        std::vector<int> ga(gall.size()), gb(gall.size());
+        for (size_t k = 0; k < gall.size(); ++k) { ga[k] = gall[k].first; gb[k] = gall[k].second; }
+        DevBuf<int> dga, dgb;
+        DevBuf<double> dq(gall.size());
+        dga.upload(ga.data(), ga.size(), st_);
+        dgb.upload(gb.data(), gb.size(), st_);
+        launch_schwarz_generic(d_shells_.p, dga.p, dgb.p, (int)gall.size(), d_tab_.p, dq.p, st_);
+        dq.download(gQall.data(), gall.size(), st_);
+        PF_HIP(hipStreamSynchronize(st_));
+    }
     const int cls_l[3][2] = {{0, 0}, {1, 0}, {1, 1}};
     // pass 1: all pairs, Schwarz bounds
     std::array<std::vector<double>, 3> Qall;
@@ -139,6 +167,7 @@ void Engine::setup_pairs() {
     PF_HIP(hipStreamSynchronize(st_));
     double qmax = 0.0;
     for (auto &q : Qall) for (double v : q) qmax = std::max(qmax, v);
+    for (double v : gQall) qmax = std::max(qmax, v);
     // pass 2: keep pairs that can pass Q_ab Q_max >= tau, sorted by Q descending
     for (int c = 0; c < 3; ++c) {
         // (Q, index) sorted by Q descending, ties by index: the order of a stable
@@ -163,6 +192,35 @@ void Engine::setup_pairs() {
         // FP32 bound rounded up so FP32 screening never drops what FP64 keeps
         for (auto &v : fq) v = std::nextafter(v, 1e30f);
         ps.f_Q.upload(fq.data(), fq.size(), st_);
+    }
+    if (!gall.empty()) {
+        // kept d pairs sorted by Q descending; kets: those pairs, then every kept s/p pair
+        std::vector<std::pair<double, int>> key;
+        for (int i = 0; i < (int)gall.size(); ++i)
+            if (gQall[i] * qmax >= schwarz_) key.push_back({gQall[i], i});
+        std::sort(key.begin(), key.end(), [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+            return x.first > y.first || (x.first == y.first && x.second < y.second);
+        });
+        std::vector<int> ksa, ksb;
+        std::vector<double> kq;
+        for (const auto &k : key) {
+            ksa.push_back(gall[k.second].first);
+            ksb.push_back(gall[k.second].second);
+            kq.push_back(k.first);
+        }
+        ngen_ = (int)ksa.size();
+        d_gsa_.upload(ksa.data(), ksa.size(), st_);
+        d_gsb_.upload(ksb.data(), ksb.size(), st_);
+        d_gQ_.upload(kq.data(), kq.size(), st_);
+        for (int c = 0; c < 3; ++c) {
+            ksa.insert(ksa.end(), pairs_[c].sa.begin(), pairs_[c].sa.end());
+            ksb.insert(ksb.end(), pairs_[c].sb.begin(), pairs_[c].sb.end());
+            kq.insert(kq.end(), pairs_[c].Q.begin(), pairs_[c].Q.end());
+        }
+        nket_gen_ = (int)ksa.size();
+        d_ksa_.upload(ksa.data(), ksa.size(), st_);
+        d_ksb_.upload(ksb.data(), ksb.size(), st_);
+        d_kQ_.upload(kq.data(), kq.size(), st_);
     }
     // One launch per unordered class.
     classes_.clear();
@@ -233,8 +291,18 @@ long long Engine::schwarz_pairs() const {
     return n;
 }
 
+GenQuartets Engine::gen_quartets() const {
+    GenQuartets q;
+    q.shells = d_shells_.p;
+    q.bsa = d_gsa_.p; q.bsb = d_gsb_.p; q.bQ = d_gQ_.p; q.nbra = ngen_;
+    q.ksa = d_ksa_.p; q.ksb = d_ksb_.p; q.kQ = d_kQ_.p; q.nket = nket_gen_; q.nsame = ngen_;
+    q.tab = d_tab_.p;
+    return q;
+}
+
 void Engine::setup_store() {
     if (store_ready_) return;
+    if (has_d_) throw Error("stored ERIs: s and p shells only (d shells run direct)");
     setup_pairs();
     const int ints[3] = {1, 3, 9};
     for (size_t q = 0; q < classes_.size(); ++q) {
@@ -335,6 +403,18 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
         a.ev1 = profile_classes ? ev[2 * q + 1] : nullptr;
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
+    if (ngen_ > 0) {
+        // quartets with a d shell, in FP64 in either precision (FP64 densities)
+        const double *Dj = D0;
+        if (uhf) {
+            dev_axpby(n2, 1.0, D0, 1.0, D1, Dj64_.p, st_);
+            Dj = Dj64_.p;
+        }
+        if constexpr (!std::is_same<R, double>::value)
+            dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, D0, D1, Dsh64_.p, dmax64_.p, st_);
+        launch_jk_generic(gen_quartets(), n, nsh, Dj, D0, uhf ? D1 : nullptr, Dsh64_.p, tau > 0 ? tau : schwarz_, J,
+                          Ka, uhf ? Kb : nullptr, st_);
+    }
     dev_assemble_g<double>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
     for (int q = 0; q < 6; ++q) prof_launches[pbase + q] += 1;
     if (profile_classes) pending_.push_back(pb);
@@ -411,6 +491,7 @@ void Engine::eri_packed_host(double *packed) {
         a.nbf = nbf(); a.nsh = nshell(); a.tab = d_tab_.p; a.packed = d.p;
         launch_jk_class<double>(B.la, B.lb, K.la, K.lb, 1, MODE_PACKED, a, st_);
     }
+    if (ngen_ > 0) launch_packed_generic(gen_quartets(), d.p, st_);
     d.download(packed, total, st_);
     PF_HIP(hipStreamSynchronize(st_));
 }
@@ -881,6 +962,7 @@ void Engine::setup_grad_pairs() {
 
 void Engine::gradient(double *grad_out, double tau, double *terms) {
     if (!have_fin_) throw Error("gradient: no converged density (the SCF did not converge, or was not run)");
+    if (has_d_) throw Error("gradient: s and p shells only (this basis has d shells)");
     setup_grad_pairs();
     const int n = nbf(), nsh = nshell(), na = mol_.natoms;
     const size_t n2 = (size_t)n * n;

@@ -12,6 +12,7 @@
 // included (no screening).
 #include "engine.h"
 #include "eri_core.h"
+#include "generic.h"
 
 namespace pf {
 
@@ -86,6 +87,7 @@ void Engine::eri_quartets(int nq, const int *quartets, double *out, long long *o
     std::vector<long long> off((size_t)nq + 1, 0);
     std::vector<int> list(4 * (size_t)std::max(nq, 1), 0);
     std::array<std::vector<int>, 9> byclass;  // canonical (l_hi, l_lo | l_hi, l_lo) classes
+    std::vector<int> gen;                      // quartets with a d shell (generic kernel)
     for (int q = 0; q < nq; ++q) {
         int l[4];
         long long n = 1;
@@ -99,6 +101,10 @@ void Engine::eri_quartets(int nq, const int *quartets, double *out, long long *o
             n *= ncart(l[k]);
         }
         off[q + 1] = off[q] + n;
+        if (std::max(std::max(l[0], l[1]), std::max(l[2], l[3])) >= 2) {
+            gen.push_back(q);
+            continue;
+        }
         const int bra = std::max(l[0], l[1]) + std::min(l[0], l[1]);  // 0 ss, 1 ps, 2 pp
         const int ket = std::max(l[2], l[3]) + std::min(l[2], l[3]);
         byclass[3 * bra + ket].push_back(q);
@@ -117,6 +123,7 @@ void Engine::eri_quartets(int nq, const int *quartets, double *out, long long *o
         sel.insert(sel.end(), byclass[c].begin(), byclass[c].end());
     }
     cbeg[9] = (int)sel.size();
+    sel.insert(sel.end(), gen.begin(), gen.end());
     d_sel.upload(sel.data(), sel.size(), st_);
     const Shell *dsh = d_shells_.p;
     const double *tab = d_tab_.p;
@@ -135,6 +142,7 @@ void Engine::eri_quartets(int nq, const int *quartets, double *out, long long *o
             default: launch_quartets<1, 1, 1, 1>(n, d_list.p, s, dsh, tab, d_off.p, d_out.p, st_); break;
         }
     }
+    launch_export_generic((int)gen.size(), d_list.p, d_sel.p + cbeg[9], dsh, tab, d_off.p, d_out.p, st_);
     PF_HIP(hipGetLastError());
     d_out.download(out, (size_t)off[nq], st_);
     PF_HIP(hipStreamSynchronize(st_));

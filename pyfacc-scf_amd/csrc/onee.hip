@@ -212,16 +212,15 @@ __global__ void __launch_bounds__(256) onee_kernel(const Shell *__restrict__ she
 #pragma unroll
     for (int k = 0; k < 9; ++k) s[k] = t[k] = v[k] = 0.0;
     int a = 0, b = 0;
-    if (valid) {
-        onee_pair_index(pid, a, b);
-        onee_pair_blocks(shells[a], shells[b], natoms, axyz, Z, lds_tab, s, t, v, sub, ONEE_LPP);
-    }
+    if (valid) onee_pair_index(pid, a, b);
+    const bool sp = valid && shells[a].l <= 1 && shells[b].l <= 1;  // pairs with a d shell: generic kernel
+    if (sp) onee_pair_blocks(shells[a], shells[b], natoms, axyz, Z, lds_tab, s, t, v, sub, ONEE_LPP);
     // every lane takes part (groups of ONEE_LPP lanes lie within one wavefront)
 #pragma unroll
     for (int k = 0; k < 9; ++k)
 #pragma unroll
         for (int o = ONEE_LPP / 2; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
-    if (valid && sub == 0) onee_store(shells[a], shells[b], nbf, s, t, v, Smat, Tmat, Vmat);
+    if (sp && sub == 0) onee_store(shells[a], shells[b], nbf, s, t, v, Smat, Tmat, Vmat);
 }
 
 // Many molecules in one launch: blockIdx.y = molecule m, whose shells start at

@@ -11,8 +11,8 @@
 namespace pf {
 
 constexpr int KMAX = 8;       // max primitives per shell (STO-6G needs 6)
-constexpr int LMAX = 1;       // s and p shells (basis.f90:76-82 handles S, P, SP only)
-constexpr int BOYS_MMAX = 12; // highest Boys order held in the table
+constexpr int LMAX = 2;       // s, p (specialised kernels) and d shells (generic kernels)
+constexpr int BOYS_MMAX = 14; // highest Boys order held in the table ((dd|dd) needs 8 + 6 Taylor terms)
 constexpr int BOYS_TAYLOR = 7;
 constexpr double BOYS_H = 1.0 / 16.0;  // table spacing in x
 constexpr double BOYS_XMAX = 40.0;     // beyond this the asymptotic form is used

@@ -80,10 +80,27 @@ def test_library_parser_matches_oracle_parser(basis):
     assert [sp[a] for a in info["atom"][:2]] == ["H", "H"]
 
 
-def test_parser_rejects_d_shells_and_missing_elements():
+@pytest.mark.parametrize("species", [["Zn"], ["Br", "H"], ["As", "H", "H", "H"], ["Se", "H", "H"]])
+def test_parser_reads_d_shells_like_the_oracle(species):
+    """D lines give Cartesian d shells (6 functions): same shell order,
+    exponents and normalised coefficients as the oracle's reader."""
+    import oracle
     import pyfacc
+    xyz = np.arange(3 * len(species), dtype=float).reshape(-1, 3)
+    info = pyfacc.basis_info(species, xyz, "sto-3g")
+    m = oracle.Molecule(species, xyz)
+    assert 2 in list(info["l"])
+    assert list(info["l"]) == [s["l"] for s in m.shells] and info["nbf"] == m.nbf
+    assert np.array_equal(info["exps"].reshape(-1), m.basis.alpha)
+    assert np.abs(info["coefs"].reshape(-1) - m.basis.coef).max() < 1e-13 * np.abs(m.basis.coef).max()
+
+
+def test_parser_rejects_f_shells_and_missing_elements(tmp_path):
+    import pyfacc
+    bad = tmp_path / "f-basis.txt"
+    bad.write_text("****\nH     0\nF    1   1.00\n      1.0000000              1.0000000\n****\n")
     with pytest.raises(pyfacc.PyfaccError, match="not supported"):
-        pyfacc.basis_info(["Zn"], np.zeros((1, 3)), "sto-3g")
+        pyfacc.basis_info(["H"], np.zeros((1, 3)), str(bad))
     with pytest.raises(pyfacc.PyfaccError, match="not found"):
         pyfacc.basis_info(["H"], np.zeros((1, 3)), "no-such-basis")
 
@@ -91,7 +108,8 @@ def test_parser_rejects_d_shells_and_missing_elements():
 def test_every_declared_symbol_is_exported():
     import pyfacc
     header = open(os.path.join(ROOT, "include", "pyfacc.h")).read()
-    names = re.findall(r"^\s*(?:void|int|const char \*)\s*\*?\s*(\w+)\s*\(", header, flags=re.M)
+    names = re.findall(r"^\s*(?:void|int|unsigned long long|const char \*)\s*\*?\s*(\w+)\s*\(", header,
+                       flags=re.M)
     assert "run_scf_c" in names and "run_scf_interface_py" in names and len(names) >= 14
     lib = ctypes.CDLL(pyfacc.LIB_PATH)
     for n in names:
