@@ -80,7 +80,9 @@ typedef struct pf_options {
     int diis_basis;       /* DIIS error vector: PF_DIIS_AO, F D S - S D F; PF_DIIS_ORTHO, X^T (F D S - S D F) X
                              with X the orthogonaliser (pf_scf and pf_scf_gradient; pf_batch_scf: AO only) */
     double mixed_anchor;  /* mixed precision: the full FP64 rebuild inside the FP32 phase comes when ||dD||_F
-                             first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient) */
+                             first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient); pf_batch_scf:
+                             FP64 builds until ||dD||_F first falls below mixed_anchor, FP32 after (0: FP32 from
+                             the start) */
     int profile;          /* 1: time each J/K class kernel with HIP events around it (pf_scf_profile's
                              milliseconds; the events add short gaps to the build), 0: no events */
     int jk_fallback;      /* testing: direct J/K builds take the paths that large systems need, at any

@@ -58,7 +58,7 @@ struct BatchDev {
     const long long *pdiis = nullptr;                             // DIIS ring offsets
     const long long *psp2 = nullptr;  // Gershgorin scratch offsets (orthogonaliser's molecule view)
     int *mactive = nullptr;   // 1 while the molecule iterates
-    int *mprec = nullptr;     // mixed form: 0 = FP32 integrals, 1 = FP64
+    int *mprec = nullptr;     // 1 = FP64 integrals; mixed form: 2 = FP64 opening, 0 = FP32 phase, then 1
     int *mhead = nullptr, *mnstore = nullptr, *muse = nullptr;  // DIIS ring head / size / extrapolate flag
     double *mcoef = nullptr;  // DIIS coefficients [M][ndiis]
     double *mB = nullptr;     // DIIS B matrices [M][ndiis][ndiis]

@@ -428,7 +428,7 @@ struct ConvArgs {
     const double *D, *Dold, *F, *H, *enuc;
     double *E, *Eold, *iters, *conv;
     int *count;
-    double eps, mixed_switch;
+    double eps, mixed_switch, mixed_open;
     int max_iter;
 };
 __global__ void __launch_bounds__(256) benergy_kernel(BatchDev b, ConvArgs a) {
@@ -456,6 +456,7 @@ __global__ void __launch_bounds__(256) benergy_kernel(BatchDev b, ConvArgs a) {
         a.Eold[m] = eel;
         a.E[m] = eel + a.enuc[m];
         bool active = true;
+        if (b.mprec[m] == 2 && errD < a.mixed_open) b.mprec[m] = 0;  // FP64 opening done: FP32 phase
         if (b.mprec[m] == 0) {
             if (errD < a.mixed_switch) { b.mprec[m] = 1; b.mnstore[m] = 0; b.mhead[m] = 0; }
         } else if (dE < a.eps && errD < a.eps) {
@@ -948,7 +949,7 @@ void BatchEngine::iteration(const pf_options &opt) {
         density_step(F_.p);
     }
     ConvArgs c{D_.p, Dold_.p, F_.p, H_.p, d_enuc_.p, d_E_.p, d_Eold_.p, d_iters_.p, d_conv_.p, d_count_.p,
-               opt.eps, opt.mixed_switch, opt.max_iter};
+               opt.eps, opt.mixed_switch, opt.mixed_anchor, opt.max_iter};
     benergy_kernel<<<M_, 256, 0, st_>>>(dev_, c);
 }
 
@@ -961,7 +962,9 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
     setup();
     const double t1 = now();
     // state: frozen flags, precision, DIIS rings, energies
-    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, opt.precision == PF_PREC_MIXED ? 0 : 1, d_mprec_.p);
+    // mixed form: FP64 opening (2) until ||dD||_F < mixed_anchor (none if 0), FP32 (0), FP64 (1)
+    const int prec0 = opt.precision != PF_PREC_MIXED ? 1 : (opt.mixed_anchor > 0.0 ? 2 : 0);
+    fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, prec0, d_mprec_.p);
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mhead_.p);
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mnstore_.p);
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_muse_.p);

@@ -79,6 +79,25 @@ def test_batch_mixed_within_bar(gpu_lib, batch64):
     assert np.abs(m["energy"] - r["energy"]).max() < 1e-6
 
 
+def test_batch_mixed_fp64_opening_uhf(gpu_lib):
+    """OH-radical clusters whose FP32-from-the-start UHF iterations land on a
+    different stationary point than FP64 (3.9e-4 Eh apart in the 4096-instance
+    sweep): with the FP64 opening phase (builds in FP64 until ||dD||_F first
+    falls below mixed_anchor = 0.1, the default) the mixed form reaches the
+    FP64 solution within the 1e-6 Eh bar."""
+    import pyfacc
+    mols = _instances([810, 1035, 1876, 1944, 2231, 2278, 2476, 2945, 3778, 3918])
+    assert all(na != nb for _, _, na, nb in mols)
+    opt = dict(eps=1e-8, max_iter=100, diis=8, schwarz=1e-10, guess=1)
+    r64 = pyfacc.batch_scf(mols, graph=True, precision=0, **opt)
+    m = pyfacc.batch_scf(mols, graph=True, precision=1, **opt)
+    assert r64["converged"].all() and m["converged"].all()
+    assert np.abs(m["energy"] - r64["energy"]).max() < 1e-6
+    # FP32 from the first build (mixed_anchor = 0) still converges
+    z = pyfacc.batch_scf(mols, graph=True, precision=1, mixed_anchor=0.0, **opt)
+    assert z["converged"].all()
+
+
 def test_batch_plain_roothaan_and_errors(gpu_lib):
     import pyfacc
     mols = _instances([3, 7, 9])  # three single waters
