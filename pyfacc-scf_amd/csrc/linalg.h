@@ -61,7 +61,15 @@ bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_
 // convergence flag (one stream synchronisation), then checks every 4 steps.
 // Returns the number of purification steps the projector needed (a good
 // first_check for the next, similar matrix is that plus one).
-size_t sp2_state_size(int n);
+// Tiles of the purification step: 16 x 16 up to n = 1024 (four times as
+// many workgroups where one step is latency-bound: 406 at n = 448), 32 x 32
+// above.  The state holds per-tile partial sums of the upper-triangle tiles.
+constexpr int sp2_tile_edge(int n) { return n <= 1024 ? 16 : 32; }
+constexpr size_t sp2_tiles(int n) {
+    return (size_t)((n + sp2_tile_edge(n) - 1) / sp2_tile_edge(n)) *
+           ((n + sp2_tile_edge(n) - 1) / sp2_tile_edge(n) + 1) / 2;
+}
+constexpr size_t sp2_state_size(int n) { return 8 + 6 * sp2_tiles(n) + 2 * (size_t)n; }
 int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
                 int max_iter = 100, int first_check = 8);
 // The same projector into P with exactly max_steps step launches, no host

@@ -3,15 +3,17 @@
 // Replaces orthonormal_diagonalize / diagonalize_symmetric (la.f90:7-57,
 // LAPACK dsyev on the host, S re-diagonalised every call) and the host
 // matmuls of rscf.f90:113-151:
-//   * gemm: FP64 MFMA (v_mfma_f64_16x16x4f64), 32x32 tile per wavefront,
-//     K staged through LDS in chunks of 16;
+//   * gemm: FP64 MFMA (v_mfma_f64_16x16x4f64), one 32x32 tile per
+//     workgroup whose wavefronts split K, partial tiles summed through LDS;
 //   * jacobi: two-sided parallel Jacobi (round-robin pairing, n/2 disjoint
 //     rotations per round, one kernel per round), used for S once and for
 //     orbital energies;
 //   * SP2 spectral projector (Niklasson, PRB 66, 155115 (2002)): the density
 //     of the lowest n_occ eigenvectors of F' = X^T F X built from GEMMs only,
 //     X <- X^2 or 2X - X^2 chosen on the device from the two traces, one
-//     kernel per step (the traces of the next iterate come from its epilogue);
+//     kernel per step over the upper-triangle tiles of X^2 (16x16 tiles up
+//     to n = 1024, 32x32 above; the traces of the next iterate come from its
+//     epilogue);
 //   * reductions (energy, ||dD||_F, traces, dot products) and element-wise
 //     Fock assembly.
 #include <algorithm>
@@ -655,8 +657,6 @@ bool inv_sqrt_spd(int n, const double *S, double *X, double *lam_min, hipStream_
 // step k-1, or by the initial kernel for k = 0) and writes slot (k+1) % 3, so
 // one kernel per step needs no separate reduction launch, and every tile sums
 // the partials in the same order (the decisions are deterministic).
-static int sp2_tiles(int n) { return ((n + 31) / 32) * ((n + 31) / 32); }
-size_t sp2_state_size(int n) { return 8 + 6 * (size_t)sp2_tiles(n) + 2 * (size_t)n; }
 
 // Tile t of the upper triangle of the nt x nt tile grid (row-major) -> (ty, tx), ty <= tx.
 __device__ __forceinline__ void tri_tile(int t, int nt, int &ty, int &tx) {
@@ -665,18 +665,19 @@ __device__ __forceinline__ void tri_tile(int t, int nt, int &ty, int &tx) {
     tx = ty + t;
 }
 
-// X0 = (hi I - F) / (hi - lo) on the upper-triangle 32x32 tiles (each written to
+// X0 = (hi I - F) / (hi - lo) on the upper-triangle T x T tiles (each written to
 // both halves, so X0 is exactly symmetric), with slot-0 partials and records.
+template <int T>
 __global__ void __launch_bounds__(256) sp2_init_kernel(int n, const double *F, double *state, double *X) {
     __shared__ double sh[4];
-    const int nt = (n + 31) / 32, tile = blockIdx.x, ntile = nt * (nt + 1) / 2;
+    const int nt = (n + T - 1) / T, tile = blockIdx.x, ntile = nt * (nt + 1) / 2;
     int ty, tx;
     tri_tile(tile, nt, ty, tx);
-    const int r0 = ty * 32, c0 = tx * 32;
+    const int r0 = ty * T, c0 = tx * T;
     const double lo = state[0], hi = state[1], sc = 1.0 / (hi - lo);
     double t1 = 0.0, t2 = 0.0;
-    for (int e = threadIdx.x; e < 1024; e += 256) {
-        const int r = r0 + (e >> 5), c = c0 + (e & 31);
+    for (int e = threadIdx.x; e < T * T; e += 256) {
+        const int r = r0 + e / T, c = c0 + e % T;
         if (r < n && c < n && (ty != tx || r <= c)) {
             const double v = ((r == c ? hi : 0.0) - F[(size_t)r * n + c]) * sc;
             X[(size_t)r * n + c] = v;
@@ -694,20 +695,87 @@ __global__ void __launch_bounds__(256) sp2_init_kernel(int n, const double *F, d
     }
 }
 
+// The 16x16 tile (r0, c0) of X X for symmetric X (n x n, row-major) by W
+// wavefronts that split k into runs of 4 KS.  Both MFMA operands are read
+// from rows of X (B[k][c] = X[c][k]), and inside a run the k index is
+// relabelled: lane (i, q) supplies k = kb + 8 (j / 2) + 2 q + (j % 2) to MFMA
+// step j, for A (row r0 + i) and B (row c0 + i) alike.  The sum over k is
+// unchanged, each lane loads pairs of doubles (16 bytes), the four lanes of a
+// row read 64 contiguous bytes per load, and every load of the run is in
+// flight before the first MFMA.  At n <= 4 KS W (448 for KS = 14, W = 8)
+// each wavefront makes one run.
+template <int KS>
+struct SymRun16 {
+    static_assert(KS % 2 == 0, "pairs of doubles per load");
+    double a[KS], b[KS];
+    // operands of the run at kb (rows and k past n read as 0)
+    __device__ __forceinline__ void load(int n, const double *__restrict__ X, int r0, int c0, int kb) {
+        const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
+        const int ra = r0 + i, rb = c0 + i;
+        const double *pa = X + (size_t)min(ra, n - 1) * n, *pb = X + (size_t)min(rb, n - 1) * n;
+        const int k0 = kb + 2 * q;
+        if ((n & 1) == 0 && kb + 4 * KS <= n) {
+            // n and k0 even: 16-byte aligned pairs, the whole run in bounds
+#pragma unroll
+            for (int m = 0; m < KS / 2; ++m) {
+                const double2 va = *reinterpret_cast<const double2 *>(pa + k0 + 8 * m);
+                const double2 vb = *reinterpret_cast<const double2 *>(pb + k0 + 8 * m);
+                a[2 * m] = va.x; a[2 * m + 1] = va.y;
+                b[2 * m] = vb.x; b[2 * m + 1] = vb.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < KS; ++j) {
+                const int k = k0 + 8 * (j >> 1) + (j & 1);
+                a[j] = k < n ? pa[k] : 0.0;
+                b[j] = k < n ? pb[k] : 0.0;
+            }
+        }
+        if (ra >= n)
+#pragma unroll
+            for (int j = 0; j < KS; ++j) a[j] = 0.0;
+        if (rb >= n)
+#pragma unroll
+            for (int j = 0; j < KS; ++j) b[j] = 0.0;
+    }
+    __device__ __forceinline__ void mma(double4_t &acc) const {
+#pragma unroll
+        for (int j = 0; j < KS; ++j) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], acc, 0, 0, 0);
+    }
+};
+
 // One purification step k: from slot k % 3 (t1 = tr X, t2 = sum X_ij^2 = tr X^2
 // for symmetric X, the previous error and done flag), X' = X^2 when
 // |t2 - nocc| < |2 t1 - t2 - nocc|, else 2X - X^2; once the idempotency error
 // |t1 - t2| is below tol (or stops shrinking below 1e-8, after step 2) the
 // step copies X and the done flag is raised.  X' partials and records go to
-// slot (k+1) % 3.  X' is symmetric: only the upper-triangle tiles are
-// computed, each written to both halves.
+// slot (k+1) % 3.  X' is symmetric: only the upper-triangle T x T tiles are
+// computed, each written to both halves.  With T = 16 the first k run's
+// operands and each thread's element of X are loaded before the step's
+// record and partials are read, so the step waits for memory once.
 constexpr int SW = GW;  // wavefronts per tile of the purification step
+constexpr int SP2_KS = 14;
+template <int T>
 __global__ void __launch_bounds__(64 * SW) sp2_step_kernel(int n, double nocc, double tol, int k, const double *X,
                                                         double *Xo, double *state) {
-    __shared__ double red[SW][16][64];
+    constexpr int NREG = T * T / 64;  // tile doubles per lane of one partial
+    __shared__ double red[SW][NREG][64];
     __shared__ double sh[SW];
-    const int nt = (n + 31) / 32, tile = blockIdx.x, ntile = nt * (nt + 1) / 2;
+    const int nt = (n + T - 1) / T, tile = blockIdx.x, ntile = nt * (nt + 1) / 2;
     const int s0 = k % 3, s1 = (k + 1) % 3;
+    int ty, tx;
+    tri_tile(tile, nt, ty, tx);
+    const int r0 = ty * T, c0 = tx * T;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    SymRun16<SP2_KS> run;
+    double xo = 0.0;  // X at this thread's element of a 16 x 16 tile
+    if constexpr (T == 16) {
+        if (4 * SP2_KS * w < n) run.load(n, X, r0, c0, 4 * SP2_KS * w);
+        if (threadIdx.x < 256) {
+            const int gr = r0 + (lane >> 4) + 4 * w, gc = c0 + (lane & 15);
+            if (gr < n && gc < n) xo = X[(size_t)gr * n + gc];
+        }
+    }
     // record [3 + 2s]: steps since convergence (0 = not converged).  The first
     // two converged steps copy X into both buffers; later ones do nothing.
     const double age = state[3 + 2 * s0];
@@ -729,24 +797,42 @@ __global__ void __launch_bounds__(64 * SW) sp2_step_kernel(int n, double nocc, d
     const double err = fabs(t1 - t2), prev = state[2 + 2 * s0];
     bool done = age != 0.0;
     if (!done && k > 2 && (err < tol || (err < 1e-8 && err >= prev))) done = true;
-    int ty, tx;
-    tri_tile(tile, nt, ty, tx);
-    const int r0 = ty * 32, c0 = tx * 32;
     const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    if (!done) mfma_tile_w<SW, 4>(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
+    if (!done) {
+        if constexpr (T == 16) {
+            double4_t acc = double4_t{0, 0, 0, 0};
+            for (int kb = 4 * SP2_KS * w; kb < n; kb += 4 * SP2_KS * SW) {
+                if (kb != 4 * SP2_KS * w) run.load(n, X, r0, c0, kb);
+                run.mma(acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[w][r][lane] = acc[r];
+            __syncthreads();
+        } else {
+            mfma_tile_w<SW, 4>(n, n, n, 0, 0, X, n, X, n, r0, c0, red);
+        }
+    }
     double u1 = 0.0, u2 = 0.0;
-    for (int idx = threadIdx.x; idx < 1024; idx += 64 * SW) {
+    for (int idx = threadIdx.x; idx < T * T; idx += 64 * SW) {
         const int reg = idx >> 6, ln = idx & 63;
         int gr, gc;
-        tile_pos(idx, r0, c0, gr, gc);
+        if constexpr (T == 16) {
+            gr = r0 + (ln >> 4) + 4 * reg;
+            gc = c0 + (ln & 15);
+        } else {
+            tile_pos(idx, r0, c0, gr, gc);
+        }
         if (gr < n && gc < n && (ty != tx || gr <= gc)) {
             const size_t o = (size_t)gr * n + gc;
+            const double x = T == 16 ? xo : X[o];
             double v;
             if (done) {
-                v = X[o];
+                v = x;
             } else {
-                const double x2 = red_sum<SW>(red, reg, ln);
-                v = square ? x2 : 2.0 * X[o] - x2;
+                double x2 = red[0][reg][ln];
+#pragma unroll
+                for (int q = 1; q < SW; ++q) x2 += red[q][reg][ln];
+                v = square ? x2 : 2.0 * x - x2;
             }
             Xo[o] = v;
             Xo[(size_t)gc * n + gr] = v;
@@ -766,19 +852,32 @@ __global__ void __launch_bounds__(64 * SW) sp2_step_kernel(int n, double nocc, d
     }
 }
 
-int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
-                int max_iter, int first_check) {
-    const int nt = (n + 31) / 32;
-    const dim3 tiles(nt * (nt + 1) / 2);  // upper-triangle tiles
-    double *rows = state + 8 + 6 * (size_t)sp2_tiles(n);
+namespace {
+// Gershgorin bounds and X0, the first part of both purification drivers.
+void sp2_begin(int n, const double *Fp, double *X, double *state, hipStream_t st) {
+    double *rows = state + 8 + 6 * sp2_tiles(n);
     gershgorin_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, Fp, rows, rows + n);
     gershgorin_bounds_kernel<<<1, 1024, 0, st>>>(n, rows, rows + n, state);
-    sp2_init_kernel<<<tiles, 256, 0, st>>>(n, Fp, state, X);
+    const dim3 tiles((unsigned)sp2_tiles(n));
+    if (sp2_tile_edge(n) == 16) sp2_init_kernel<16><<<tiles, 256, 0, st>>>(n, Fp, state, X);
+    else sp2_init_kernel<32><<<tiles, 256, 0, st>>>(n, Fp, state, X);
+}
+
+void sp2_step(int n, double nocc, double tol, int k, const double *X, double *Xo, double *state, hipStream_t st) {
+    const dim3 tiles((unsigned)sp2_tiles(n));
+    if (sp2_tile_edge(n) == 16) sp2_step_kernel<16><<<tiles, 64 * SW, 0, st>>>(n, nocc, tol, k, X, Xo, state);
+    else sp2_step_kernel<32><<<tiles, 64 * SW, 0, st>>>(n, nocc, tol, k, X, Xo, state);
+}
+}  // namespace
+
+int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, double *state, hipStream_t st,
+                int max_iter, int first_check) {
+    sp2_begin(n, Fp, X, state, st);
     const double tol = 1e-13 * std::max(1.0, nocc);
     double *cur = X, *nxt = work;
     int used = max_iter, check = std::max(1, std::min(first_check, max_iter));
     for (int it = 0; it < max_iter; ++it) {
-        sp2_step_kernel<<<tiles, 64 * SW, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
+        sp2_step(n, nocc, tol, it, cur, nxt, state, st);
         std::swap(cur, nxt);
         if (it + 1 == check) {
             // record written by step it: steps since convergence (0: not yet)
@@ -811,16 +910,11 @@ int sp2_density(int n, const double *Fp, double nocc, double *X, double *work, d
 // P.  Nothing here synchronises, so the sequence can be captured in a graph.
 void sp2_project_fixed(int n, const double *Fp, double nocc, double *P, double *X, double *work, double *state,
                        int max_steps, hipStream_t st) {
-    const int nt = (n + 31) / 32;
-    const dim3 tiles(nt * (nt + 1) / 2);
-    double *rows = state + 8 + 6 * (size_t)sp2_tiles(n);
-    gershgorin_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, Fp, rows, rows + n);
-    gershgorin_bounds_kernel<<<1, 1024, 0, st>>>(n, rows, rows + n, state);
-    sp2_init_kernel<<<tiles, 256, 0, st>>>(n, Fp, state, X);
+    sp2_begin(n, Fp, X, state, st);
     const double tol = 1e-13 * std::max(1.0, nocc);
     double *cur = X, *nxt = work;
     for (int it = 0; it < max_steps; ++it) {
-        sp2_step_kernel<<<tiles, 64 * SW, 0, st>>>(n, nocc, tol, it, cur, nxt, state);
+        sp2_step(n, nocc, tol, it, cur, nxt, state, st);
         std::swap(cur, nxt);
     }
     const dim3 g2((n + 127) / 128, n);

@@ -21,9 +21,7 @@ namespace {
 
 constexpr int NOALLOC_NMAX = PF_PROJECTOR_NOALLOC_NMAX;
 // X and work (n x n each) and the SP2 state at the largest n
-constexpr size_t SCRATCH_DOUBLES = 2 * (size_t)NOALLOC_NMAX * NOALLOC_NMAX + 8 +
-                                   6 * (size_t)((NOALLOC_NMAX + 31) / 32) * ((NOALLOC_NMAX + 31) / 32) +
-                                   2 * (size_t)NOALLOC_NMAX;
+constexpr size_t SCRATCH_DOUBLES = 2 * (size_t)NOALLOC_NMAX * NOALLOC_NMAX + sp2_state_size(NOALLOC_NMAX);
 
 // One copy per GPU, held until the process ends.  Host code never uses this
 // name as an address: the device address comes from hipGetSymbolAddress.

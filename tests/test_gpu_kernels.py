@@ -48,7 +48,9 @@ def test_jacobi_eigh(gpu_lib, n):
     assert np.abs(A @ V - V * w).max() < 1e-11 * max(1.0, np.abs(wr).max())
 
 
-@pytest.mark.parametrize("n,nocc", [(7, 5), (40, 13), (112, 56)])
+# 16 x 16 tiles up to n = 1024 (odd n: the guarded scalar loads; 448: one k
+# run per wavefront; 1000: several), 32 x 32 tiles above
+@pytest.mark.parametrize("n,nocc", [(7, 5), (40, 13), (112, 56), (449, 200), (448, 320), (1000, 380), (1030, 500)])
 def test_sp2_projector(gpu_lib, n, nocc):
     import pyfacc
     rng = np.random.default_rng(n + nocc)
