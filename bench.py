@@ -79,19 +79,25 @@ def cpu_baseline(species, xyz, n_builds, target_s):
     rng = np.random.default_rng(0)
     C = np.linalg.qr(rng.standard_normal((m.nbf, m.nbf)))[0][:, :nocc]
     D = 2.0 * C @ C.T
-    def timed(stride):
+    def timed(stride, phase=0):
         t0 = time.perf_counter()
-        _, _, done = m.jk_direct(D, thresh=1e-10, Q=Q, stride=stride)
+        _, _, done = m.jk_direct(D, thresh=1e-10, Q=Q, stride=stride, phase=phase)
         return time.perf_counter() - t0, done
 
     # loop overhead of visiting every pair of pairs (no integrals), then a
-    # calibration sample, then a sample sized to ~target_s seconds
+    # calibration sample, then two disjoint samples (every stride-th quartet
+    # from offsets 0 and stride/2) of ~target_s/2 seconds each: their mean
+    # is the full-build estimate, their difference its sampling spread
     t_loop, _ = timed(1 << 62)
     t_cal, done = timed(512)
     full_est = max(t_cal - t_loop, 1e-3) * 512
-    stride = max(2, int(np.ceil(full_est / target_s)))
-    t_cal, done = timed(stride)
-    t_build = max(t_cal - t_loop, 1e-9) * stride + t_loop
+    stride = max(2, int(np.ceil(2.0 * full_est / target_s)))
+    t_a, done_a = timed(stride, 0)
+    t_b, done_b = timed(stride, stride // 2)
+    t_cal, done = t_a + t_b, done_a + done_b
+    b_a = max(t_a - t_loop, 1e-9) * stride + t_loop
+    b_b = max(t_b - t_loop, 1e-9) * stride + t_loop
+    t_build = 0.5 * (b_a + b_b)
     # the same build on one thread, on a sparser sample (about target_s / 4);
     # its pair-loop overhead is taken as the all-thread one times the threads
     oracle.set_threads(1)
@@ -112,10 +118,13 @@ def cpu_baseline(species, xyz, n_builds, target_s):
         "unit": "SCF/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle direct J/K (McMurchie-Davidson, OpenMP {threads} threads) on every {stride}-th "
-                   f"screened shell quartet ({done} quartets, {t_cal:.1f} s); full build = {t_build:.1f} s, "
+        "sample": (f"oracle direct J/K (McMurchie-Davidson, OpenMP {threads} threads) on two disjoint samples, "
+                   f"every {stride}-th screened shell quartet from offsets 0 and {stride // 2} ({done} quartets, "
+                   f"{t_cal:.1f} s); full build = {t_build:.1f} s (the two samples: {b_a:.1f} s, {b_b:.1f} s), "
                    f"x {n_builds} builds of the FP64 GPU SCF"),
         "fock_build_s": t_build,
+        "fock_build_samples_s": [b_a, b_b],
+        "sampling_spread": abs(b_a - b_b) / t_build,
         "host": {"nproc": os.cpu_count(), "cpu_model": model},
         "one_thread": {"value": 1.0 / (t_build1 * n_builds), "fock_build_s": t_build1,
                        "sample": f"every {stride1}-th screened shell quartet ({done1} quartets, {t1:.1f} s)",

@@ -219,14 +219,17 @@ class Molecule:
         lib().or_jk_packed(ctypes.c_int(self.nbf), _p(packed), ctypes.c_int(nd), _p(D), _p(J), _p(K))
         return J, K
 
-    def jk_direct(self, D, thresh=0.0, Q=None, stride=1):
+    def jk_direct(self, D, thresh=0.0, Q=None, stride=1, phase=0):
+        """Direct J/K; with stride > 1 only the shell quartets whose loop index
+        is phase modulo stride (a timing sample)."""
         D = np.ascontiguousarray(D, dtype=np.float64)
         nd = 1 if D.ndim == 2 else D.shape[0]
         if Q is None:
             Q = self.schwarz()
         J, K = np.zeros_like(D), np.zeros_like(D)
         done = lib().or_jk_direct(*self.basis.args(), _p(Q), ctypes.c_double(thresh), ctypes.c_int(nd), _p(D),
-                                  _p(J), _p(K), ctypes.c_int64(stride))
+                                  _p(J), _p(K), ctypes.c_int64(stride),
+                                  ctypes.c_int64(phase % max(stride, 1)))
         return J, K, done
 
     def scf(self, nalpha, nbeta, eps=1e-10, max_iter=100, eri_mode=0, schwarz=0.0, diis=0, force_uhf=False):

@@ -577,10 +577,11 @@ void or_schwarz(int nsh, const int *l, const int *np, const double *ctr, const d
 /* Direct J/K over unique shell quartets with Schwarz screening
  * (Q_ab Q_cd < thresh skipped) and a shell-quartet subset selector for
  * timing samples: quartet q (in loop order) is processed iff
- * q % stride == 0.  Returns the number of shell quartets processed. */
+ * q % stride == phase (0 <= phase < stride; disjoint samples for different
+ * phases).  Returns the number of shell quartets processed. */
 int64_t or_jk_direct(int nsh, const int *l, const int *np, const double *ctr, const double *alpha,
                      const double *coef, const double *Q, double thresh, int ndens, const double *D,
-                     double *J, double *K, int64_t stride)
+                     double *J, double *K, int64_t stride, int64_t phase)
 {
     or_basis bs = {nsh, l, np, ctr, alpha, coef};
     int *off = (int *)malloc(sizeof(int) * nsh), nbf;
@@ -605,7 +606,7 @@ int64_t or_jk_direct(int nsh, const int *l, const int *np, const double *ctr, co
                 int c = pa[r], d = pb[r];
                 if (Q[a * nsh + b] * Q[c * nsh + d] < thresh) continue;
                 int64_t qid = (int64_t)p * (p + 1) / 2 + r;
-                if (stride > 1 && qid % stride) continue;
+                if (stride > 1 && qid % stride != phase) continue;
                 ++done;
                 shell_eri(&bs, a, b, c, d, blk);
                 double deg = 1.0;
@@ -852,7 +853,7 @@ int or_scf(int nsh, const int *l, const int *np, const double *ctr, const double
     for (it = 1; it <= max_iter; ++it) {
         if (uhf) for (size_t q = 0; q < n2; ++q) Dt[q] = D[q] + D[n2 + q];
         if (eri_mode == 0) or_jk_packed(n, packed, nd, D, J, K);
-        else or_jk_direct(nsh, l, np, ctr, alpha, coef, Q, schwarz, nd, D, J, K, 1);
+        else or_jk_direct(nsh, l, np, ctr, alpha, coef, Q, schwarz, nd, D, J, K, 1, 0);
         ++builds;
         if (!uhf) {
             for (size_t q = 0; q < n2; ++q) F[q] = H[q] + J[q] - 0.5 * K[q];

@@ -236,6 +236,23 @@ def test_jk_matches_full_tensor_and_direct():
     assert np.abs(J2 - Jr).max() < 1e-12 and np.abs(K2 - Kr).max() < 1e-12
 
 
+def test_jk_direct_timing_samples_partition_the_build():
+    """The CPU-baseline samples (every stride-th shell quartet from a given
+    offset) for all offsets of one stride are disjoint and sum to the whole
+    build, quartet counts included."""
+    sp, xyz = load_xyz("h2o_c1.xyz")
+    sp, xyz = sp * 3, np.concatenate([xyz, xyz + [0.0, 0.0, 5.5], xyz + [5.5, 0.0, 0.0]])
+    m = oracle.Molecule(sp, xyz)
+    rng = np.random.default_rng(9)
+    D = rng.standard_normal((m.nbf, m.nbf))
+    D = D + D.T
+    J, K, done = m.jk_direct(D, thresh=1e-10)
+    parts = [m.jk_direct(D, thresh=1e-10, stride=3, phase=ph) for ph in range(3)]
+    assert sum(p[2] for p in parts) == done and min(p[2] for p in parts) > 0
+    assert np.abs(sum(p[0] for p in parts) - J).max() < 1e-12
+    assert np.abs(sum(p[1] for p in parts) - K).max() < 1e-12
+
+
 def test_jacobi_matches_lapack():
     rng = np.random.default_rng(3)
     M = rng.standard_normal((23, 23))
