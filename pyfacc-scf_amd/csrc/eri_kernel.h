@@ -57,18 +57,36 @@ __device__ __forceinline__ long long quad_index(int i, int j, int k, int l) {
 // of total angular momentum >= 3.
 template <int LA, int LB, int LC, int LD>
 constexpr int jk_waves() { return LA + LB + LC + LD >= 3 ? 4 : 8; }
+// Direct-mode workgroup shape per class and precision (measured on (H2O)64
+// STO-3G): FP32 (ps|ss) runs twelve wavefronts per workgroup held to 80
+// registers (two workgroups: six wavefronts per SIMD; 13 % faster than
+// eight wavefronts at five), the others keep jk_waves and the compiler's
+// register choice.
+template <int LA, int LB, int LC, int LD, typename R>
+constexpr int jk_direct_waves() {
+    return LA + LB + LC + LD == 1 && std::is_same<R, float>::value ? 12 : jk_waves<LA, LB, LC, LD>();
+}
+// minimum wavefronts per SIMD the register allocation must allow (1: no limit)
+template <int LA, int LB, int LC, int LD, typename R>
+constexpr int jk_direct_occupancy() {
+    return LA + LB + LC + LD == 1 && std::is_same<R, float>::value ? 6 : 1;
+}
+// Ket-primitive prefetch in eri_block: 2-4 % faster for every class but FP64
+// (ss|ss), where its registers cost a wavefront per SIMD (16 % slower).
+template <int LA, int LB, int LC, int LD, typename R>
+constexpr bool jk_prefetch() { return LA + LB + LC + LD != 0 || !std::is_same<R, double>::value; }
 // Kets per direct-mode work item (one workgroup): a few passes of every
 // wavefront, so that the per-item cost (queue atomic, row flush) is small
 // beside the item's integrals while items stay short enough to balance.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr int jk_chunk() {
     constexpr int L = LA + LB + LC + LD;
-    return 64 * jk_waves<LA, LB, LC, LD>() * (L <= 1 ? 8 : (L <= 3 ? 4 : 2));
+    return 64 * jk_direct_waves<LA, LB, LC, LD, R>() * (L <= 1 ? 8 : (L <= 3 ? 4 : 2));
 }
 
 // Contracted (ab|cd) block, eri[((ia*NB+ib)*NC+ic)*ND+id], of bra pair i
 // (uniform; its primitive data through bprim) and ket pair j (this lane).
-template <int LA, int LB, int LC, int LD, typename R>
+template <int LA, int LB, int LC, int LD, typename R, bool PREFETCH = true>
 __device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restrict__ bprim, int i, R ABx, R ABy,
                                           R ABz, const PairView<R> &K, int j, const R *lds_boys, R *eri) {
     using E = Eri<LA, LB, LC, LD, R>;
@@ -80,19 +98,30 @@ __device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restri
     // the cut: ket primitives are sorted by bound, so the loop stops there)
     const int kpk = K.kpn[j], kpb = __builtin_amdgcn_readfirstlane(B.kpn[i]);
     const R bmax = bprim[(size_t)PF_BND * B.npad + i], cut = B.peps;  // bra primitive 0 has the largest bound
-    int nu = 0;
-    for (int u = 0; u < kpk; ++u, ++nu) {
+    // this lane's ket primitive fields, the next primitive's loaded while the
+    // current one runs through the bra primitives (hides the load latency)
+    struct KetPrim {
+        R bnd, q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, o2q;
+    };
+    const auto load = [&](int u) {
         const R *kq = K.prim + (size_t)u * PF_NFIELD * K.npad + j;
-        if (kq[PF_BND * K.npad] * bmax < cut) break;
-        const R q = kq[PF_P * K.npad], Qx = kq[PF_PX * K.npad], Qy = kq[PF_PY * K.npad], Qz = kq[PF_PZ * K.npad];
-        const R QCx = kq[PF_AX * K.npad], QCy = kq[PF_AY * K.npad], QCz = kq[PF_AZ * K.npad],
-                Kk = kq[PF_K * K.npad], o2q = kq[PF_O2P * K.npad];
+        return KetPrim{kq[PF_BND * K.npad], kq[PF_P * K.npad],  kq[PF_PX * K.npad], kq[PF_PY * K.npad],
+                       kq[PF_PZ * K.npad],  kq[PF_AX * K.npad], kq[PF_AY * K.npad], kq[PF_AZ * K.npad],
+                       kq[PF_K * K.npad],   kq[PF_O2P * K.npad]};
+    };
+    int nu = 0;
+    KetPrim nx = (PREFETCH && kpk > 0) ? load(0) : KetPrim{};
+    for (int u = 0; u < kpk; ++u, ++nu) {
+        const KetPrim c = PREFETCH ? nx : load(u);
+        if (c.bnd * bmax < cut) break;
+        if (PREFETCH && u + 1 < kpk) nx = load(u + 1);
         for (int v = 0; v < kpb; ++v) {
             // bra data: read-only in these kernels and wavefront-uniform -> scalar loads
             const R *bp = bprim + (size_t)v * PF_NFIELD * B.npad + i;
             E::template prim<true>(bp[PF_P * B.npad], bp[PF_O2P * B.npad], bp[PF_PX * B.npad], bp[PF_PY * B.npad],
                                    bp[PF_PZ * B.npad], bp[PF_AX * B.npad], bp[PF_AY * B.npad], bp[PF_AZ * B.npad],
-                                   bp[PF_K * B.npad], q, o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, lds_boys, acc);
+                                   bp[PF_K * B.npad], c.q, c.o2q, c.Qx, c.Qy, c.Qz, c.QCx, c.QCy, c.QCz, c.Kk,
+                                   lds_boys, acc);
         }
     }
     const R CDx = K.AB[j], CDy = K.AB[K.npad + j], CDz = K.AB[2 * K.npad + j];
@@ -331,9 +360,9 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
 // g.krows) and flushed to device memory after the item; with g.krows = 0
 // (rows too long for LDS) the exchange terms go straight to device memory.
 template <int LA, int LB, int LC, int LD, typename R, int NK>
-__global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
+__global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (jk_direct_occupancy<LA, LB, LC, LD, R>()))
     jk_direct_kernel(JKArgs<R> g, const R *__restrict__ bprim, const int *__restrict__ item_off) {
-    constexpr int W = jk_waves<LA, LB, LC, LD>(), BLOCK = 64 * W;
+    constexpr int W = jk_direct_waves<LA, LB, LC, LD, R>(), BLOCK = 64 * W;
     using E = Eri<LA, LB, LC, LD, R>;
     using BT = BoysLds<E::L, R>;
     constexpr int NA = E::NA, NB = E::NB, NINT = E::NINT, NAB = NA + NB;
@@ -432,7 +461,8 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>()))
             const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
             ++counted;
             R eri[NINT];
-            pcounted += eri_block<LA, LB, LC, LD, R>(B, bprim, i, ABx, ABy, ABz, K, j, lds_boys, eri);
+            pcounted += eri_block<LA, LB, LC, LD, R, jk_prefetch<LA, LB, LC, LD, R>()>(B, bprim, i, ABx, ABy, ABz, K, j,
+                                                                                   lds_boys, eri);
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
             double jcd[NCD];  // this quartet's J'_cd block
             const auto jadd = [&](int row, int col, R v) { jcd[(row - oc) * E::ND + (col - od)] = double(v); };
@@ -585,12 +615,12 @@ __global__ void __launch_bounds__(256) jcd_reduce_kernel(JKArgs<R> g, const int 
 
 template <int LA, int LB, int LC, int LD, typename R, int NK>
 void launch_direct(const JKArgs<R> &args, hipStream_t st) {
-    constexpr int W = jk_waves<LA, LB, LC, LD>();
+    constexpr int W = jk_direct_waves<LA, LB, LC, LD, R>();
     constexpr int NA = nc(LA), NB = nc(LB);
     if (!args.work || !args.item_off || !args.nket || !args.joff)
         throw Error("jk: direct mode needs its work-plan buffers");
     JKArgs<R> a = args;
-    a.chunk = jk_chunk<LA, LB, LC, LD>();
+    a.chunk = jk_chunk<LA, LB, LC, LD, R>();
     const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK>;
     // launch shape per (device, nbf), worked out once per thread: the
     // attribute and occupancy queries cost host time on every build otherwise
