@@ -89,6 +89,11 @@ typedef struct pf_options {
                              size: bit 0, exchange rows added with device atomics (rows too long for LDS);
                              bit 1, J'_cd added with device atomics (ket lists beyond the J'_cd buffer).
                              0: chosen by size (default) */
+    int deterministic;    /* 1: every J', K' accumulation of a direct build is an int64 fixed-point add
+                             (scale 2^e from a bound on the sums, resolution <= ~1e-13 at 448 basis functions),
+                             so repeated SCFs give bit-identical energies and densities; s and p shells, direct
+                             builds only (pf_scf, pf_scf_gradient's SCF; not pf_batch_scf).  0: FP64 atomics
+                             (repeated runs differ by ~1e-11) */
 } pf_options;
 
 typedef struct pf_result {
@@ -114,7 +119,7 @@ typedef struct pf_result {
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
  * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error,
- * mixed_anchor 0.1, profile 0, jk_fallback 0. */
+ * mixed_anchor 0.1, profile 0, jk_fallback 0, deterministic 0. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

@@ -111,6 +111,7 @@ void pf_default_options(pf_options *o) {
     o->mixed_anchor = 0.1;
     o->profile = 0;
     o->jk_fallback = 0;
+    o->deterministic = 0;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }
@@ -232,6 +233,7 @@ int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int
         if (nmol <= 0) throw Error("nmol must be positive");
         if (o.level_shift != 0.0 || o.diis_basis != PF_DIIS_AO)
             throw Error("level_shift and diis_basis are not available in the batched engine (pf_batch_scf)");
+        if (o.deterministic) throw Error("deterministic is not available in the batched engine (pf_batch_scf)");
         if (!natoms || !nalpha || !nbeta) throw Error("null natoms, nalpha or nbeta");
         std::vector<BatchMolecule> mols(nmol);
         size_t a0 = 0;

@@ -98,6 +98,7 @@ class Engine {
     std::array<int, 24> prof_class_l{};  // per slot: la lb lc ld as launched (bra | ket)
     bool profile_classes = false;
     int jk_fallback = 0;  // pf_options::jk_fallback: bit 0 K rows, bit 1 J'_cd through device atomics
+    bool deterministic = false;  // pf_options::deterministic: order-independent (int64 fixed-point) J/K sums
 
    private:
     void setup_one_electron();
@@ -147,6 +148,8 @@ class Engine {
     GenQuartets gen_quartets() const;
     // J/K accumulators
     DevBuf<double> J64_, K64a_, K64b_, Dsh64_, dmax64_;
+    DevBuf<double> fx_;   // deterministic builds: the fixed-point scale (device scalar)
+    double qmax_ = 0.0;   // largest Schwarz bound of any shell pair
     DevBuf<float> Dsh32_, dmax32_, Dj32_, Dk32a_, Dk32b_;
     DevBuf<double> Dj64_;
     DevBuf<unsigned long long> count_;  // [precision][quartets 6, primitive quartets 6]

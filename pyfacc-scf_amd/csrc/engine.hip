@@ -168,6 +168,7 @@ void Engine::setup_pairs() {
     double qmax = 0.0;
     for (auto &q : Qall) for (double v : q) qmax = std::max(qmax, v);
     for (double v : gQall) qmax = std::max(qmax, v);
+    qmax_ = qmax;
     // pass 2: keep pairs that can pass Q_ab Q_max >= tau, sorted by Q descending
     for (int c = 0; c < 3; ++c) {
         // (Q, index) sorted by Q descending, ties by index: the order of a stable
@@ -378,6 +379,13 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
         a.Dsh = Dsh32_.p; a.dmax = dmax32_.p;
     }
     a.J = J; a.K0 = Ka; a.K1 = uhf ? Kb : nullptr;
+    if (deterministic) {
+        if (mode != MODE_DIRECT) throw Error("deterministic J/K: direct builds only (eri_mode 0)");
+        if (ngen_ > 0) throw Error("deterministic J/K: s and p shells only");
+        if (!fx_.p) fx_.alloc(1);
+        dev_fx_scale((int64_t)n2, D0, D1, qmax_, fx_.p, st_);
+        a.fx = fx_.p;
+    }
     const int pbase = std::is_same<R, double>::value ? 0 : 6;
     PendingBuild pb{pbase, {}};
     hipEvent_t *ev = pb.ev;
@@ -414,6 +422,11 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
             dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, D0, D1, Dsh64_.p, dmax64_.p, st_);
         launch_jk_generic(gen_quartets(), n, nsh, Dj, D0, uhf ? D1 : nullptr, Dsh64_.p, tau > 0 ? tau : schwarz_, J,
                           Ka, uhf ? Kb : nullptr, st_);
+    }
+    if (deterministic) {
+        dev_fx_to_double((int64_t)n2, fx_.p, J, st_);
+        dev_fx_to_double((int64_t)n2, fx_.p, Ka, st_);
+        if (uhf) dev_fx_to_double((int64_t)n2, fx_.p, Kb, st_);
     }
     dev_assemble_g<double>(n, J, Ka, uhf ? Kb : nullptr, G0, G1, uhf, accumulate, st_);
     for (int q = 0; q < 6; ++q) prof_launches[pbase + q] += 1;
@@ -634,6 +647,7 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         throw Error("stored ERIs are FP64 only");
     if (opt.jk_fallback < 0 || opt.jk_fallback > 3) throw Error("jk_fallback must be 0..3");
     jk_fallback = opt.jk_fallback;
+    deterministic = opt.deterministic != 0;
 
     setup_one_electron();
     w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); state_.alloc(sp2_state_size(n));

@@ -50,6 +50,16 @@ __device__ __forceinline__ long long quad_index(int i, int j, int k, int l) {
     return ij >= kl ? ij * (ij + 1) / 2 + kl : kl * (kl + 1) / 2 + ij;
 }
 
+// One addition into a J'/K' accumulator (LDS or device memory): an FP64
+// atomic, or (DET) the int64 atomic add of round(v s) on the same 8 bytes.
+template <bool DET>
+__device__ __forceinline__ void acc_add(double *p, double v, double s) {
+    if constexpr (DET)
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(__double2ll_rn(v * s)));
+    else
+        atomicAdd(p, v);
+}
+
 }  // namespace
 
 // Wavefronts per workgroup: eight (two per SIMD, sharing one LDS Boys table)
@@ -359,7 +369,7 @@ __global__ void __launch_bounds__((64 * jk_waves<LA, LB, LC, LD>())) jk_kernel(J
 // every density and its J'_ab are summed in LDS (dynamic shared memory, when
 // g.krows) and flushed to device memory after the item; with g.krows = 0
 // (rows too long for LDS) the exchange terms go straight to device memory.
-template <int LA, int LB, int LC, int LD, typename R, int NK>
+template <int LA, int LB, int LC, int LD, typename R, int NK, bool DET>
 __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (jk_direct_occupancy<LA, LB, LC, LD, R>()))
     jk_direct_kernel(JKArgs<R> g, const R *__restrict__ bprim, const int *__restrict__ item_off) {
     constexpr int W = jk_direct_waves<LA, LB, LC, LD, R>(), BLOCK = 64 * W;
@@ -383,6 +393,7 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
     const int total = g.work[1];
     constexpr int NCD = E::NC * E::ND;
     const bool jstore = g.jbuf != nullptr && g.work[2] != 0;
+    const double fxs = DET ? *g.fx : 0.0;
     unsigned long long counted = 0, pcounted = 0;
     __syncthreads();
     while (true) {
@@ -469,14 +480,14 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
             if (g.krows) {
                 digest<LA, LB, LC, LD, R, NK>(dens_view(g), eri, oa, ob, oc, od, deg, dab, jab, jadd,
                                               [&](int kk, int r, int col, R v) {
-                                                  atomicAdd(&lds_acc[(kk * NAB + r) * nbf + col], double(v));
+                                                  acc_add<DET>(&lds_acc[(kk * NAB + r) * nbf + col], double(v), fxs);
                                               });
             } else {
                 digest<LA, LB, LC, LD, R, NK>(dens_view(g), eri, oa, ob, oc, od, deg, dab, jab, jadd,
                                               [&](int kk, int r, int col, R v) {
                                                   double *Ko = kk == 0 ? g.K0 : g.K1;
                                                   const int row = r < NA ? oa + r : ob + r - NA;
-                                                  atomicAdd(&Ko[(size_t)row * nbf + col], double(v));
+                                                  acc_add<DET>(&Ko[(size_t)row * nbf + col], double(v), fxs);
                                               });
             }
             if (jstore) {
@@ -486,27 +497,38 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
             } else {
 #pragma unroll
                 for (int q = 0; q < NCD; ++q)
-                    atomicAdd(&g.J[(size_t)(oc + q / E::ND) * nbf + od + q % E::ND], jcd[q]);
+                    acc_add<DET>(&g.J[(size_t)(oc + q / E::ND) * nbf + od + q % E::ND], jcd[q], fxs);
             }
         }
 #pragma unroll
         for (int ab = 0; ab < NA * NB; ++ab) {
             const double s = wave_sum(double(jab[ab]));
-            if (lane == 0) atomicAdd(&ljab[ab], s);
+            if (lane == 0) acc_add<DET>(&ljab[ab], s, fxs);
         }
         __syncthreads();
-        // flush the item's rows and J'_ab; entries are left zero for the next item
+        // flush the item's rows and J'_ab (partial sums: doubles, or with DET
+        // int64 in the same bytes); entries are left zero for the next item
         for (int e = threadIdx.x; e < nrow + NA * NB; e += BLOCK) {
-            const double v = lds_acc[e];
-            if (v == 0.0) continue;
-            lds_acc[e] = 0.0;
+            double *dst;
             if (e < nrow) {
                 const int kk = e / (NAB * nbf), rc = e - kk * NAB * nbf, r = rc / nbf, col = rc - r * nbf;
                 const int row = r < NA ? oa + r : ob + r - NA;
-                atomicAdd(&(kk == 0 ? g.K0 : g.K1)[(size_t)row * nbf + col], v);
+                dst = &(kk == 0 ? g.K0 : g.K1)[(size_t)row * nbf + col];
             } else {
                 const int ab = e - nrow;
-                atomicAdd(&g.J[(size_t)(oa + ab / NB) * nbf + ob + ab % NB], v);
+                dst = &g.J[(size_t)(oa + ab / NB) * nbf + ob + ab % NB];
+            }
+            if constexpr (DET) {
+                unsigned long long *li = reinterpret_cast<unsigned long long *>(lds_acc);
+                const unsigned long long v = li[e];
+                if (v == 0) continue;
+                li[e] = 0;
+                atomicAdd(reinterpret_cast<unsigned long long *>(dst), v);
+            } else {
+                const double v = lds_acc[e];
+                if (v == 0.0) continue;
+                lds_acc[e] = 0.0;
+                atomicAdd(dst, v);
             }
         }
         __syncthreads();
@@ -598,7 +620,7 @@ __global__ void __launch_bounds__(256) jk_jrange_kernel(JKArgs<R> g, int *jhi) {
     jhi[j] = lo;
 }
 constexpr int JRED_TILE = 256;
-template <int NC, int ND, typename R>
+template <int NC, int ND, typename R, bool DET>
 __global__ void __launch_bounds__(256) jcd_reduce_kernel(JKArgs<R> g, const int *jhi) {
     constexpr int NCD = NC * ND;
     if (!g.work[2]) return;
@@ -610,10 +632,10 @@ __global__ void __launch_bounds__(256) jcd_reduce_kernel(JKArgs<R> g, const int 
     double s = 0.0;
 #pragma unroll 8
     for (int i = i0; i < i1; ++i) s += g.jbuf[(g.joff[i] + j) * NCD + k];
-    if (s != 0.0) atomicAdd(&g.J[(size_t)(g.ket.oa[j] + k / ND) * g.nbf + g.ket.ob[j] + k % ND], s);
+    if (s != 0.0) acc_add<DET>(&g.J[(size_t)(g.ket.oa[j] + k / ND) * g.nbf + g.ket.ob[j] + k % ND], s, DET ? *g.fx : 0.0);
 }
 
-template <int LA, int LB, int LC, int LD, typename R, int NK>
+template <int LA, int LB, int LC, int LD, typename R, int NK, bool DET>
 void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     constexpr int W = jk_direct_waves<LA, LB, LC, LD, R>();
     constexpr int NA = nc(LA), NB = nc(LB);
@@ -621,7 +643,7 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
         throw Error("jk: direct mode needs its work-plan buffers");
     JKArgs<R> a = args;
     a.chunk = jk_chunk<LA, LB, LC, LD, R>();
-    const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK>;
+    const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK, DET>;
     // launch shape per (device, nbf), worked out once per thread: the
     // attribute and occupancy queries cost host time on every build otherwise
     struct Shape {
@@ -663,7 +685,7 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
         const long long nt = (long long)a.ket.n * NCD;
         jk_jrange_kernel<R><<<(a.ket.n + 255) / 256, 256, 0, st>>>(a, a.jhi);
         const dim3 grid((unsigned)((nt + 255) / 256), (a.bra.n + JRED_TILE - 1) / JRED_TILE);
-        jcd_reduce_kernel<nc(LC), nc(LD), R><<<grid, 256, 0, st>>>(a, a.jhi);
+        jcd_reduce_kernel<nc(LC), nc(LD), R, DET><<<grid, 256, 0, st>>>(a, a.jhi);
     }
 }
 
@@ -697,8 +719,13 @@ void launch_nk_mode(int nk, int mode, const JKArgs<R> &args, hipStream_t st) {
         if (args.ev1) PF_HIP(hipEventRecord(args.ev1, st));
         return;
     }
-    if (nk == 1) launch_direct<LA, LB, LC, LD, R, 1>(args, st);
-    else launch_direct<LA, LB, LC, LD, R, 2>(args, st);
+    if (args.fx) {
+        if (nk == 1) launch_direct<LA, LB, LC, LD, R, 1, true>(args, st);
+        else launch_direct<LA, LB, LC, LD, R, 2, true>(args, st);
+    } else {
+        if (nk == 1) launch_direct<LA, LB, LC, LD, R, 1, false>(args, st);
+        else launch_direct<LA, LB, LC, LD, R, 2, false>(args, st);
+    }
 }
 
 }  // namespace pf

@@ -82,6 +82,12 @@ struct JKArgs {
     long long jcap = 0;
     int krows = 0;  // direct mode: bra exchange rows summed in LDS
     int katomic = 0;  // direct mode: force exchange rows to device atomics (the no-LDS-rows path)
+    // direct mode, deterministic accumulation: device scalar 2^s.  Every
+    // addition into J', K' (LDS rows and device memory) is then the integer
+    // add of round(v 2^s) to the accumulator's 8 bytes read as a two's-
+    // complement int64, so the sums do not depend on the order of the adds.
+    // nullptr: FP64 atomics.
+    const double *fx = nullptr;
     // host side only: when set, recorded on the stream right before and after
     // the class's main ERI kernel (its duration without the helper kernels)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

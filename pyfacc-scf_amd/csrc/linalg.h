@@ -35,6 +35,14 @@ void dev_energy_rhf(int n, const double *D, const double *H, const double *F, do
 void dev_energy_uhf(int n, const double *Da, const double *Db, const double *H, const double *Fa, const double *Fb,
                     double *out, double *part, hipStream_t st);
 void dev_fill(int64_t n, double v, double *x, hipStream_t st);
+// Deterministic J/K accumulation (JKArgs::fx): *fx = 2^e with e the largest
+// exponent for which 2 qmax^2 (sum |D0| + sum |D1|) 2^e < 2^61, a bound on
+// every partial sum of a J', K' accumulator (|(ab|cd)| <= Q_ab Q_cd <= qmax^2),
+// so the int64 sums cannot overflow; D1 may be null.  One workgroup, fixed
+// summation order.
+void dev_fx_scale(int64_t n, const double *D0, const double *D1, double qmax, double *fx, hipStream_t st);
+// x[i] (int64 fixed point with scale *fx, in place) -> double
+void dev_fx_to_double(int64_t n, const double *fx, double *x, hipStream_t st);
 void dev_fillf(int64_t n, float v, float *x, hipStream_t st);
 template <typename R>
 void dev_assemble_g(int n, const R *J, const R *K0, const R *K1, double *G0, double *G1, bool uhf, bool accumulate,

@@ -253,6 +253,37 @@ void dev_fill(int64_t n, double v, double *x, hipStream_t st) {
     fill_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, v, x);
     PF_HIP(hipGetLastError());
 }
+__global__ void __launch_bounds__(1024) fx_scale_kernel(int64_t n, const double *D0, const double *D1, double qmax,
+                                                         double *fx) {
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) s += fabs(D0[i]) + (D1 ? fabs(D1[i]) : 0.0);
+    s = block_sum<16>(s, sh);
+    if (threadIdx.x == 0) {
+        int e = 0;
+        (void)frexp(2.0 * qmax * qmax * s, &e);  // bound < 2^e
+        fx[0] = ldexp(1.0, min(61 - e, 200));
+    }
+}
+
+__global__ void fx_to_double_kernel(int64_t n, const double *fx, double *x) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long long v = reinterpret_cast<const long long *>(x)[i];
+    x[i] = double(v) * (1.0 / fx[0]);
+}
+
+void dev_fx_scale(int64_t n, const double *D0, const double *D1, double qmax, double *fx, hipStream_t st) {
+    fx_scale_kernel<<<1, 1024, 0, st>>>(n, D0, D1, qmax, fx);
+    PF_HIP(hipGetLastError());
+}
+
+void dev_fx_to_double(int64_t n, const double *fx, double *x, hipStream_t st) {
+    if (n <= 0) return;
+    fx_to_double_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, fx, x);
+    PF_HIP(hipGetLastError());
+}
+
 void dev_fillf(int64_t n, float v, float *x, hipStream_t st) {
     if (n <= 0) return;
     fillf_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, v, x);
