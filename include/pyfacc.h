@@ -90,10 +90,11 @@ typedef struct pf_options {
                              bit 1, J'_cd added with device atomics (ket lists beyond the J'_cd buffer).
                              0: chosen by size (default) */
     int deterministic;    /* 1: every J', K' accumulation of a direct build is an int64 fixed-point add
-                             (scale 2^e from a bound on the sums, resolution <= ~1e-13 at 448 basis functions),
-                             so repeated SCFs give bit-identical energies and densities; s and p shells, direct
-                             builds only (pf_scf, pf_scf_gradient's SCF; not pf_batch_scf).  0: FP64 atomics
-                             (repeated runs differ by ~1e-11) */
+                             (scale 2^e with B 2^e < 2^61 for the bound B = 2 Qmax^2 sum|D| on the sums; the
+                             resolution 2^-e is below 1e-12 while B < 2^21), so repeated SCFs give bit-identical
+                             energies and densities; s and p shells, direct builds only (pf_scf,
+                             pf_scf_gradient's SCF; not pf_batch_scf).  0: FP64 atomics (repeated runs differ
+                             by ~1e-11) */
 } pf_options;
 
 typedef struct pf_result {
