@@ -81,9 +81,26 @@ struct Eri {
         if constexpr (LDS) BoysLds<L, R>::eval(T, tab, F);
         else boys_eval<L, R>(T, tab, F);
         const R pref = Kb * Kk * r;
+        if constexpr (L == 0) {
+            acc[0] = fma(pref, F[0], acc[0]);  // (ss|ss)
+            return;
+        } else if constexpr (LAB == 1 && LCD == 0) {
+            // (ps|ss): [p_i|s] = pref (PA_i F_0 + WP_i F_1), WP_i = -(rho/p) PQ_i,
+            // accumulated as two FMAs per component
+            const R a = pref * F[0], b = (pref * rop) * F[1];
+            acc[0] = fma(PAx, a, acc[0]);
+            acc[1] = fma(PAy, a, acc[1]);
+            acc[2] = fma(PAz, a, acc[2]);
+            acc[0] = fma(-PQ[0], b, acc[0]);
+            acc[1] = fma(-PQ[1], b, acc[1]);
+            acc[2] = fma(-PQ[2], b, acc[2]);
+            return;
+        }
+        // the recurrences are linear in [s|s]^m: run them on F_m and scale the
+        // results by pref as they are accumulated (one FMA each)
         R th[NTH];
 #pragma unroll
-        for (int m = 0; m <= L; ++m) th[m] = pref * F[m];
+        for (int m = 0; m <= L; ++m) th[m] = F[m];
         if constexpr (LAB > 0) {
             const R PA[3] = {PAx, PAy, PAz};
             const R WP[3] = {-rop * PQ[0], -rop * PQ[1], -rop * PQ[2]};
@@ -129,7 +146,7 @@ struct Eri {
 #pragma unroll
         for (int e = 0; e < NE; ++e)
 #pragma unroll
-            for (int f = 0; f < NF; ++f) acc[e * NF + f] += th[base(E0 + e, F0 + f)];
+            for (int f = 0; f < NF; ++f) acc[e * NF + f] = fma(pref, th[base(E0 + e, F0 + f)], acc[e * NF + f]);
     }
 
     // Contracted [e0|f0] -> (ab|cd), out[((ia*NB+ib)*NC+ic)*ND+id].
