@@ -111,8 +111,8 @@ typedef struct pf_result {
     int nshell;
     double t_total;       /* seconds, wall clock of the SCF call */
     double t_setup;       /* one-electron integrals, S^-1/2, pair lists, Schwarz, stored ERIs */
-    double t_fock;        /* J/K builds */
-    double t_diag;        /* density from Fock (orthogonalise + SP2/Jacobi + back-transform) */
+    double t_fock;        /* J/K builds (pf_options.profile = 1; 0 otherwise) */
+    double t_diag;        /* density from Fock (orthogonalise + SP2/Jacobi + back-transform; profile = 1) */
     double quartets;      /* shell quartets evaluated over all builds */
     double min_overlap_eig; /* smallest eigenvalue of S (la.f90:40-44 warns below 1e-8) */
 } pf_result;

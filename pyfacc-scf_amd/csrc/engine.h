@@ -115,8 +115,13 @@ class Engine {
     // Synchronises the stream, adds the builds since the last call to the
     // prof_* statistics and returns their shell-quartet count.
     unsigned long long collect_profile();
-    void density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
-                           double *eps_dev);
+    // D from F (eps_dev: orbital energies by Jacobi as well).  With defer_slot
+    // >= 0 and a step count from an earlier projector, SP2 runs a fixed number
+    // of steps with no host round trip and returns false: its convergence
+    // record is copied to sp2_rec_[defer_slot] for the caller to read with the
+    // iteration's energy (and to redo this density if it is unset).
+    bool density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
+                           double *eps_dev, int defer_slot = -1);
     // Superposition of atomic densities (spin-summed, host, nbf^2 row-major).
     std::vector<double> sad_density();
 
@@ -170,8 +175,11 @@ class Engine {
     std::array<DevBuf<long long>, 6> store_off_;
     std::array<DevBuf<int>, 6> store_nket_;
     // work
-    DevBuf<double> w1_, w2_, w3_, state_;
-    int sp2_hint_ = 7;  // purification steps the last projector needed
+    DevBuf<double> w1_, w2_, w3_, w4_, state_;
+    DevBuf<double> sp2_rec_;      // deferred purifications: convergence record per spin
+    int sp2_hint_ = 7;            // purification steps the last projector needed
+    bool sp2_hint_set_ = false;   // sp2_hint_ comes from a finished projector
+    int sp2_defer_steps_ = 0;     // steps of the last deferred purification
     // converged densities of the last SCF (for the gradient)
     DevBuf<double> Dfin_;
     bool have_fin_ = false, uhf_fin_ = false;

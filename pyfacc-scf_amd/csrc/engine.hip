@@ -511,8 +511,8 @@ void Engine::eri_packed_host(double *packed) {
 
 // D = occ * X P X^T with P the projector onto the nocc lowest eigenvectors
 // of F' = X^T F X (la.f90:52-54 and rscf.f90:140 / uscf.f90:183-184).
-void Engine::density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
-                               double *eps_dev) {
+bool Engine::density_from_fock(const double *F, int nocc, double occ, double *D, const pf_options &opt,
+                               double *eps_dev, int defer_slot) {
     const int n = nbf();
     gemm_f64(n, false, false, 1.0, F, X_.p, 0.0, w1_.p, st_);   // F X
     gemm_f64(n, true, false, 1.0, X_.p, w1_.p, 0.0, w2_.p, st_);  // X^T F X
@@ -522,7 +522,7 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
             DevBuf<double> C(size_t(n) * n);
             jacobi_eigh(n, w2_.p, eps_dev, C.p, st_);
         }
-        return;
+        return true;
     }
     auto by_eigenvectors = [&]() {
         DevBuf<double> w(n), C(size_t(n) * n);
@@ -532,9 +532,21 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
     };
     if (opt.diag == PF_DIAG_JACOBI || eps_dev) {
         by_eigenvectors();
-        return;
+        return true;
     }
-    constexpr int SP2_MAX = 100;
+    constexpr int SP2_MAX = 100, SP2_SLACK = 3;
+    if (defer_slot >= 0 && sp2_hint_set_) {
+        // the last projector's step count plus a margin, no host round trip
+        const int nsteps = std::min(SP2_MAX, sp2_hint_ + SP2_SLACK);
+        sp2_project_fixed(n, w2_.p, double(nocc), w3_.p, w1_.p, w4_.p, state_.p, nsteps, st_);
+        // record of the last step: steps since convergence (0: not converged)
+        PF_HIP(hipMemcpyAsync(sp2_rec_.p + defer_slot, state_.p + 3 + 2 * (nsteps % 3), sizeof(double),
+                              hipMemcpyDeviceToDevice, st_));
+        sp2_defer_steps_ = nsteps;
+        gemm_f64(n, false, false, 1.0, X_.p, w3_.p, 0.0, w1_.p, st_);  // X P
+        gemm_f64(n, false, true, occ, w1_.p, X_.p, 0.0, D, st_);       // occ X P X^T
+        return false;
+    }
     const int used = sp2_density(n, w2_.p, double(nocc), w3_.p, w1_.p, state_.p, st_, SP2_MAX, sp2_hint_ + 1);
     if (used >= SP2_MAX) {
         // No projector: the occupied/virtual boundary falls inside a degenerate
@@ -543,11 +555,13 @@ void Engine::density_from_fock(const double *F, int nocc, double occ, double *D,
         // summation-order noise would break the symmetry.  The eigenvector
         // route occupies whole orbitals.
         by_eigenvectors();
-        return;
+        return true;
     }
     sp2_hint_ = used;
+    sp2_hint_set_ = true;
     gemm_f64(n, false, false, 1.0, X_.p, w3_.p, 0.0, w1_.p, st_);       // X P
     gemm_f64(n, false, true, occ, w1_.p, X_.p, 0.0, D, st_);            // occ X P X^T
+    return true;
 }
 
 // Spin-summed density of the isolated atom of element `el` (atomic number Z)
@@ -650,7 +664,8 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     deterministic = opt.deterministic != 0;
 
     setup_one_electron();
-    w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); state_.alloc(sp2_state_size(n));
+    w1_.alloc(n2); w2_.alloc(n2); w3_.alloc(n2); w4_.alloc(n2); state_.alloc(sp2_state_size(n));
+    sp2_rec_.alloc(2);
     // orthogonaliser with X X^T = S^{-1}, computed once: S^{-1/2} by Newton-Schulz,
     // or the reference's X = U s^{-1/2} (la.f90:36-50) from the Jacobi solver
     // when that iteration does not converge
@@ -689,8 +704,10 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         // core guess (rscf.f90:113-114; uscf.f90:140-144, C_alpha = C_beta)
         const double td = now();
         for (int s = 0; s < nd; ++s) density_from_fock(H_.p, nocc[s], occ, D.p + s * n2, opt, nullptr);
-        PF_HIP(hipStreamSynchronize(st_));
-        res.t_diag += now() - td;
+        if (opt.profile) {
+            PF_HIP(hipStreamSynchronize(st_));
+            res.t_diag += now() - td;
+        }
     }
     dev_fill(n2 * nd, 0.0, Dbuilt.p, st_);
     dev_fill(n2 * nd, 0.0, G.p, st_);
@@ -713,8 +730,9 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
     int it = 0, since_full = 0;
     bool anchored = false;
     bool conv = false;
-    // per iteration: events at the start of the Fock build, its end, and the end
-    // of the density/energy step (t_fock and t_diag without host round trips)
+    // per iteration, when profiling: events at the start of the Fock build, its
+    // end, and the end of the density/energy step (t_fock and t_diag without
+    // host round trips)
     std::vector<std::array<hipEvent_t, 3>> tev;
     struct ReleaseEvents {  // back to the pool on every exit, also when the SCF throws
         std::vector<std::array<hipEvent_t, 3>> &v;
@@ -733,6 +751,9 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
         gemm_f64(n, false, false, 1.0, w1_.p, X_.p, 0.0, e, st_);
     };
     auto mark = [&](int k) {
+        // per-iteration events only when profiling: reading 3 per iteration back
+        // costs ~0.8 ms of host time at the end of a 16-iteration SCF
+        if (!opt.profile) return;
         if (k == 0) {
             tev.emplace_back();
             for (auto &e : tev.back()) e = event_acquire();
@@ -870,15 +891,43 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
             }
             Fuse = Fls.p;
         }
-        for (int s = 0; s < nd; ++s) density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr);
         // energy with the new D and the F built this iteration (rscf.f90:144-151; uscf.f90:186-193)
-        if (!uhf) dev_energy_rhf(n, D.p, H_.p, F.p, red.p, part.p, st_);
-        else dev_energy_uhf(n, D.p, D.p + n2, H_.p, F.p, F.p + n2, red.p, part.p, st_);
-        dev_diff_sq(n2 * nd, D.p, Dold.p, red.p + 1, part.p, st_);
+        const auto energy_and_dd = [&]() {
+            if (!uhf) dev_energy_rhf(n, D.p, H_.p, F.p, red.p, part.p, st_);
+            else dev_energy_uhf(n, D.p, D.p + n2, H_.p, F.p, F.p + n2, red.p, part.p, st_);
+            dev_diff_sq(n2 * nd, D.p, Dold.p, red.p + 1, part.p, st_);
+        };
+        bool deferred = false, dfr[2] = {false, false};
+        for (int s = 0; s < nd; ++s) {
+            dfr[s] = !density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr, s);
+            deferred = deferred || dfr[s];
+        }
+        energy_and_dd();
         mark(2);
-        double hv[2];
+        double hv[2], rec[2] = {1.0, 1.0};
         PF_HIP(hipMemcpyAsync(hv, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+        if (deferred) PF_HIP(hipMemcpyAsync(rec, sp2_rec_.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st_));
         PF_HIP(hipStreamSynchronize(st_));
+        if (deferred) {
+            // a purification that had not converged in its fixed step count: this
+            // iteration's densities again with the checked purification (and its
+            // eigenvector fallback), then the energy and ||dD|| again
+            bool redo = false;
+            int used = 0;
+            for (int s = 0; s < nd; ++s) {
+                if (!dfr[s]) continue;
+                if (rec[s] == 0.0) redo = true;
+                else used = std::max(used, sp2_defer_steps_ - (int)rec[s]);
+            }
+            if (redo) {
+                for (int s = 0; s < nd; ++s) density_from_fock(Fuse + s * n2, nocc[s], occ, D.p + s * n2, opt, nullptr);
+                energy_and_dd();
+                PF_HIP(hipMemcpyAsync(hv, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+                PF_HIP(hipStreamSynchronize(st_));
+            } else {
+                sp2_hint_ = std::max(used, 1);
+            }
+        }
         Eel = uhf ? hv[0] : 0.5 * hv[0];
         const double dE_ = std::fabs(Eel - Eold), errD = std::sqrt(hv[1]);
         last_dD = errD;

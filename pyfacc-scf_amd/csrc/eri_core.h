@@ -84,9 +84,10 @@ struct Eri {
         if constexpr (L == 0) {
             acc[0] = fma(pref, F[0], acc[0]);  // (ss|ss)
             return;
-        } else if constexpr (LAB == 1 && LCD == 0) {
+        } else if constexpr (LA == 1 && LB == 0 && LCD == 0) {
             // (ps|ss): [p_i|s] = pref (PA_i F_0 + WP_i F_1), WP_i = -(rho/p) PQ_i,
-            // accumulated as two FMAs per component
+            // accumulated as two FMAs per component (acc holds [p|s] only; for
+            // (sp|ss), E0 = 0, acc holds [s|s] and [p|s] and takes the general path)
             const R a = pref * F[0], b = (pref * rop) * F[1];
             acc[0] = fma(PAx, a, acc[0]);
             acc[1] = fma(PAy, a, acc[1]);
