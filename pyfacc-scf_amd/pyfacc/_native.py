@@ -217,10 +217,12 @@ def scf(species, coords_bohr, nalpha, nbeta, basis="sto-3g", options=None, want_
     """Run an SCF; returns a dict of pf_result fields (+ D, eps on request)."""
     opt = options if options is not None else default_options(**kw)
     m = _Mol(species, coords_bohr)
-    n = basis_info(species, coords_bohr, basis, bool(opt.renormalize))["nbf"]
     nd = 1 if nalpha == nbeta else 2
-    D = np.zeros((nd, n, n)) if want_density else None
-    eps = np.zeros((nd, n)) if want_eps else None
+    D = eps = None
+    if want_density or want_eps:  # the basis size only for the output arrays (host work per call otherwise)
+        n = basis_info(species, coords_bohr, basis, bool(opt.renormalize))["nbf"]
+        D = np.zeros((nd, n, n)) if want_density else None
+        eps = np.zeros((nd, n)) if want_eps else None
     r = Result()
     _check(load_library().pf_scf(*m.args(), basis.encode(), int(nalpha), int(nbeta), ctypes.byref(opt),
                                  ctypes.byref(r), _ptr(D), _ptr(eps)))
