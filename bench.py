@@ -243,6 +243,9 @@ def batched(pyfacc, count):
         "guess": "SAD",
         "hipgraph": bool(st64["graph"]),
         "digest_waves_autotuned": int(st64["digest_waves"]),
+        "digest_item_values_autotuned": float(st64["digest_item_values"]),
+        "digest_tail_waves_autotuned": int(st64["digest_tail_waves"]),
+        "digest_tail_item_values_autotuned": float(st64["digest_tail_item_values"]),
     }
 
 

@@ -214,10 +214,16 @@ typedef struct {
     double store_bytes; /* device bytes of the stored integrals */
     int iterations;     /* batched iterations run (the slowest molecule's count) */
     int graph;          /* 1: iterations replayed from one captured hipGraph */
-    int digest_waves;   /* autotuned wavefronts per molecule in the J/K digest */
+    int digest_waves;   /* autotuned wavefronts per workgroup in the J/K digest while more than half the
+                           molecules iterate */
     int nproblems;      /* spin densities in the batch (1 per RHF, 2 per UHF molecule) */
     int sp2_fallbacks;  /* densities formed from eigenvectors (aufbau) because the purification did not
                            converge: the occupied/virtual boundary inside a degenerate level */
+    int digest_tail_waves; /* the same after at least half the molecules have converged */
+    double digest_item_values; /* autotuned work item of the digest while more than half the molecules
+                                  iterate: stored integrals per item (a larger molecule is split over
+                                  several workgroups; 1e18: none is) */
+    double digest_tail_item_values; /* the same after at least half the molecules have converged */
 } pf_batch_stats;
 
 /* Batched SCF over many independent molecules in one set of launches.  No

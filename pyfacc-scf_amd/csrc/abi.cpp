@@ -256,6 +256,9 @@ int pf_batch_scf(int nmol, const int32_t *natoms, const char *species, const int
             stats->iterations = bs.iterations; stats->graph = bs.graph; stats->digest_waves = bs.digest_waves;
             stats->nproblems = bs.nproblems;
             stats->sp2_fallbacks = bs.sp2_fallbacks;
+            stats->digest_tail_waves = bs.digest_tail_waves;
+            stats->digest_item_values = bs.digest_item_values;
+            stats->digest_tail_item_values = bs.digest_tail_item_values;
         }
     });
 }

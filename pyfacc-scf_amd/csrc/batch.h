@@ -70,7 +70,8 @@ struct BatchStats {
     double t_total = 0, t_setup = 0, t_eri = 0, t_scf = 0;
     double eri_values = 0, store_bytes = 0;
     int iterations = 0, graph = 0, digest_waves = 0, nproblems = 0;
-    int sp2_fallbacks = 0;
+    int sp2_fallbacks = 0, digest_tail_waves = 0;
+    double digest_item_values = 0, digest_tail_item_values = 0;
 };
 
 class BatchEngine {
@@ -88,7 +89,11 @@ class BatchEngine {
     void setup_orthogonalizer();
     void density_step(const double *F);  // D from F (per problem) for the active molecules
     void iteration(const pf_options &opt);
-    void digest(int waves);
+    void digest(int set);
+    // work items of digest shape `set`: molecules whose stored integrals
+    // exceed item_values split into about equal parts (whole bra pairs),
+    // largest items first
+    void build_items(int set, double item_values);
 
     hipStream_t st_ = nullptr;
     std::vector<BatchMolecule> mols_;
@@ -125,14 +130,22 @@ class BatchEngine {
     std::array<DevBuf<long long>, 6> soff_;
     std::array<DevBuf<int>, 6> snket_, skbase_, sbr_;
     std::array<BatchClass, 6> cls_;
-    DevBuf<BatchItem> items_;
-    DevBuf<int> d_msplit_;
+    // two digest shapes (work items + wavefronts per workgroup): [0] while
+    // more than half the molecules iterate, [1] for the rest of the run
+    struct ItemSet {
+        DevBuf<BatchItem> items;
+        DevBuf<int> msplit;  // per molecule: 1 if split over several items
+        int n = 0, waves = 0;
+        double values = 0.0;  // work-item size the items were built with
+    };
+    std::array<ItemSet, 2> sets_;
+    int cur_set_ = 0;
     DevBuf<long long> d_gacc_off_;
     DevBuf<double> gacc_;
-    int nitems_ = 0;
+    std::array<std::vector<double>, 6> work_;  // per class: stored values per bra pair
+    std::array<std::vector<int>, 6> brq_;      // per class: bra range of each molecule
     BatchDev dev_;
     double eri_values_ = 0;
-    int tuned_waves_ = 0;
 };
 
 }  // namespace pf

@@ -732,8 +732,8 @@ void BatchEngine::setup() {
     // quartet plan per class (bra class x >= ket class y) and the integral store
     PF_HIP(hipStreamSynchronize(st_));
     const double t0 = now();
-    std::array<std::vector<double>, 6> work;  // stored values per bra pair
-    std::array<std::vector<int>, 6> brq;      // per class: bra range of each molecule
+    auto &work = work_;  // stored values per bra pair
+    auto &brq = brq_;    // per class: bra range of each molecule
     int q = 0;
     for (int x = 0; x < 3; ++x)
         for (int y = 0; y <= x; ++y, ++q) {
@@ -789,51 +789,9 @@ void BatchEngine::setup() {
         }
     PF_HIP(hipStreamSynchronize(st_));
     t_eri_ = now() - t0;
-    // digest work items: a molecule whose stored integrals exceed ITEM_VALUES
-    // is split into about equal parts (whole bra pairs), so that a few large
-    // molecules spread over many CUs; items go out largest first
-    {
-        constexpr double ITEM_VALUES = 262144.0;
-        std::vector<BatchItem> items;
-        std::vector<double> iwork;
-        std::vector<int> msplit(M_, 0);
-        std::vector<long long> goff(M_);
-        long long gtot = 0;
-        for (int m = 0; m < M_; ++m) {
-            goff[m] = gtot;
-            gtot += 3LL * mn_[m] * (mn_[m] + 1) / 2;
-            double wm = 0.0;
-            for (int c = 0; c < 6; ++c)
-                for (int i = brq[c][2 * m]; i < brq[c][2 * m + 1]; ++i) wm += work[c][i];
-            const int G = std::max(1, (int)std::ceil(wm / ITEM_VALUES));
-            msplit[m] = G > 1;
-            int sq = 0, si = brq[0][2 * m], g = 0;
-            double cum = 0.0, icum = 0.0;
-            for (int c = 0; c < 6 && G > 1; ++c)
-                for (int i = brq[c][2 * m]; i < brq[c][2 * m + 1]; ++i) {
-                    cum += work[c][i];
-                    icum += work[c][i];
-                    if (g < G - 1 && cum >= wm * (g + 1) / G) {
-                        items.push_back({m, sq, si, c, i + 1, 1});
-                        iwork.push_back(icum);
-                        sq = c; si = i + 1; icum = 0.0; ++g;
-                    }
-                }
-            items.push_back({m, sq, si, 5, brq[5][2 * m + 1], G > 1 ? 1 : 0});
-            iwork.push_back(G > 1 ? icum : wm);
-        }
-        std::vector<int> order(items.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return iwork[x] > iwork[y]; });
-        std::vector<BatchItem> sorted;
-        for (int k : order) sorted.push_back(items[k]);
-        nitems_ = (int)sorted.size();
-        items_.upload(sorted.data(), sorted.size(), st_);
-        d_msplit_.upload(msplit.data(), M_, st_);
-        d_gacc_off_.upload(goff.data(), M_, st_);
-        gacc_.alloc(std::max(gtot, 1LL));
-        dev_fill(gacc_.n, 0.0, gacc_.p, st_);
-        dev_.gacc_off = d_gacc_off_.p;
+    for (int s = 0; s < 2; ++s) {  // the autotuner may rebuild them with another shape
+        build_items(s, 262144.0);
+        sets_[s].waves = 8;
     }
     phase("quartet plan + stored ERIs");
     batch_digest_prepare(nmax_);
@@ -925,15 +883,67 @@ void BatchEngine::density_step(const double *F) {
     bdtot_kernel<<<g, 256, 0, st_>>>(dev_, D_.p, Dtot_.p);
 }
 
-void BatchEngine::digest(int waves) {
-    launch_batch_digest(dev_, cls_, items_.p, nitems_, d_msplit_.p, D_.p, Dtot_.p, H_.p, F_.p, gacc_.p, waves, st_);
+void BatchEngine::build_items(int set, double item_values) {
+    ItemSet &S = sets_[set];
+    if (item_values == S.values) return;
+    S.values = item_values;
+    const auto &work = work_;
+    const auto &brq = brq_;
+    std::vector<BatchItem> items;
+    std::vector<double> iwork;
+    std::vector<int> msplit(M_, 0);
+    std::vector<long long> goff(M_);
+    long long gtot = 0;
+    for (int m = 0; m < M_; ++m) {
+        goff[m] = gtot;
+        gtot += 3LL * mn_[m] * (mn_[m] + 1) / 2;
+        double wm = 0.0;
+        for (int c = 0; c < 6; ++c)
+            for (int i = brq[c][2 * m]; i < brq[c][2 * m + 1]; ++i) wm += work[c][i];
+        const int G = std::max(1, (int)std::ceil(wm / item_values));
+        msplit[m] = G > 1;
+        int sq = 0, si = brq[0][2 * m], g = 0;
+        double cum = 0.0, icum = 0.0;
+        for (int c = 0; c < 6 && G > 1; ++c)
+            for (int i = brq[c][2 * m]; i < brq[c][2 * m + 1]; ++i) {
+                cum += work[c][i];
+                icum += work[c][i];
+                if (g < G - 1 && cum >= wm * (g + 1) / G) {
+                    items.push_back({m, sq, si, c, i + 1, 1});
+                    iwork.push_back(icum);
+                    sq = c; si = i + 1; icum = 0.0; ++g;
+                }
+            }
+        items.push_back({m, sq, si, 5, brq[5][2 * m + 1], G > 1 ? 1 : 0});
+        iwork.push_back(G > 1 ? icum : wm);
+    }
+    std::vector<int> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return iwork[x] > iwork[y]; });
+    std::vector<BatchItem> sorted;
+    for (int k : order) sorted.push_back(items[k]);
+    S.n = (int)sorted.size();
+    S.items.upload(sorted.data(), sorted.size(), st_);
+    S.msplit.upload(msplit.data(), M_, st_);
+    if (!gacc_.p) {
+        d_gacc_off_.upload(goff.data(), M_, st_);
+        gacc_.alloc(std::max(gtot, 1LL));
+        dev_fill(gacc_.n, 0.0, gacc_.p, st_);
+        dev_.gacc_off = d_gacc_off_.p;
+    }
+    PF_HIP(hipStreamSynchronize(st_));
+}
+
+void BatchEngine::digest(int set) {
+    const ItemSet &S = sets_[set];
+    launch_batch_digest(dev_, cls_, S.items.p, S.n, S.msplit.p, D_.p, Dtot_.p, H_.p, F_.p, gacc_.p, S.waves, st_);
 }
 
 // One SCF iteration for every active molecule (no host decision inside).
 void BatchEngine::iteration(const pf_options &opt) {
     const dim3 g = pgrid(P_, nmax_), gt(P_, ntile(nmax_));
     fill_int_kernel<<<1, 64, 0, st_>>>(1, 0, d_count_.p);
-    digest(tuned_waves_);
+    digest(cur_set_);
     bcopy_kernel<<<g, 256, 0, st_>>>(dev_, D_.p, d_poff_.p, Dold_.p);
     if (opt.diis > 1) {
         BGemm a{F_.p, D_.p, nullptr, W1_.p, d_poff_.p, d_poff_.p, d_poff_.p, d_poff_.p, nullptr, 0, 0, 1.0, 0.0};
@@ -987,57 +997,110 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
         bcopy_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, H_.p, d_pmoff_.p, Fx_.p);
         density_step(Fx_.p);
     }
-    // autotune the digest's wavefronts per molecule (cached per device and batch shape)
+    // autotune the digest's two shapes -- wavefronts per workgroup x stored
+    // integrals per work item -- on the real batch; cached per device and
+    // batch shape.  Shape 0 is timed with every molecule active (the first
+    // iterations); shape 1 with only the 2 % of molecules with the most
+    // stored integrals active, the state after most molecules have
+    // converged, where one large molecule in one workgroup sets the time.
     hipDeviceProp_t prop;
     int dev = 0;
     PF_HIP(hipGetDevice(&dev));
     PF_HIP(hipGetDeviceProperties(&prop, dev));
-    const std::string key = std::string("batch_digest_waves:") + prop.gcnArchName + ":" + std::to_string(M_) + ":" +
+    const std::string key = std::string("batch_digest_shapes:") + prop.gcnArchName + ":" + std::to_string(M_) + ":" +
                             std::to_string(nmax_);
+    constexpr int NWAVES = 2, NITEM = 4, NSHAPE = NWAVES * NITEM;
+    const int waves_c[NWAVES] = {4, 8};
+    const double item_c[NITEM] = {65536.0, 262144.0, 1048576.0, 1e18};  // 1e18: no molecule split
+    auto set_shape = [&](int set, int c) {
+        sets_[set].waves = waves_c[c / NITEM];
+        build_items(set, item_c[c % NITEM]);
+    };
     auto cache = tune_read();
-    if (cache.count(key) && (cache[key] == 4 || cache[key] == 8)) {
-        tuned_waves_ = cache[key];
+    const int code = cache.count(key) ? cache[key] : -1;
+    if (code >= 0 && code < NSHAPE * NSHAPE) {
+        set_shape(0, code / NSHAPE);
+        set_shape(1, code % NSHAPE);
     } else {
         hipEvent_t e0, e1;
         PF_HIP(hipEventCreate(&e0));
         PF_HIP(hipEventCreate(&e1));
-        float best = 1e30f;
-        for (int w : {4, 8}) {
-            digest(w);  // warm
+        std::vector<double> wm(M_, 0.0);
+        for (int c = 0; c < 6; ++c)
+            for (int m = 0; m < M_; ++m)
+                for (int i = brq_[c][2 * m]; i < brq_[c][2 * m + 1]; ++i) wm[m] += work_[c][i];
+        std::vector<double> ws = wm;
+        std::sort(ws.begin(), ws.end());
+        const double wcut = ws[(size_t)(0.98 * (M_ - 1))];
+        std::vector<int> tail(M_);
+        for (int m = 0; m < M_; ++m) tail[m] = wm[m] >= wcut ? 1 : 0;
+        DevBuf<int> d_tail;
+        d_tail.upload(tail.data(), M_, st_);
+        auto time_digest = [&]() {
             PF_HIP(hipEventRecord(e0, st_));
-            digest(w);
+            digest(0);
             PF_HIP(hipEventRecord(e1, st_));
             PF_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
             PF_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; tuned_waves_ = w; }
-        }
+            return ms;
+        };
+        // three rounds over all shapes, each shape's best round counts
+        std::vector<float> t_all(NSHAPE, 1e30f), t_tail(NSHAPE, 1e30f);
+        for (int round = 0; round < 3; ++round)
+            for (int c = 0; c < NSHAPE; ++c) {
+                set_shape(0, c);
+                if (round == 0) digest(0);  // warm
+                t_all[c] = std::min(t_all[c], time_digest());
+                PF_HIP(hipMemcpyAsync(d_mactive_.p, d_tail.p, sizeof(int) * M_, hipMemcpyDeviceToDevice, st_));
+                t_tail[c] = std::min(t_tail[c], time_digest());
+                fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 1, d_mactive_.p);
+            }
+        PF_HIP(hipStreamSynchronize(st_));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        cache[key] = tuned_waves_;
+        int best0 = 0, best1 = 0;
+        for (int c = 0; c < NSHAPE; ++c) {
+            if (verbose_)
+                std::fprintf(stderr, "batch digest: %d waves, items of %.0f values: %.3f ms all, %.3f ms tail\n",
+                             waves_c[c / NITEM], item_c[c % NITEM], t_all[c], t_tail[c]);
+            if (t_all[c] < t_all[best0]) best0 = c;
+            if (t_tail[c] < t_tail[best1]) best1 = c;
+        }
+        set_shape(0, best0);
+        set_shape(1, best1);
+        cache[key] = best0 * NSHAPE + best1;
         tune_write(cache);
     }
     PF_HIP(hipStreamSynchronize(st_));
     const double t2 = now();
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    if (use_graph) {
-        PF_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-        iteration(opt);
-        PF_HIP(hipStreamEndCapture(st_, &graph));
-        PF_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
+    // one graph per digest shape (one if both shapes are the same)
+    const bool two = sets_[0].waves != sets_[1].waves || sets_[0].values != sets_[1].values;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    if (use_graph)
+        for (int s = 0; s < (two ? 2 : 1); ++s) {
+            cur_set_ = s;
+            PF_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+            iteration(opt);
+            PF_HIP(hipStreamEndCapture(st_, &graph[s]));
+            PF_HIP(hipGraphInstantiate(&exec[s], graph[s], nullptr, nullptr, 0));
+        }
+    cur_set_ = 0;
     int it = 0;
     for (; it < opt.max_iter; ++it) {
-        if (use_graph) PF_HIP(hipGraphLaunch(exec, st_));
+        if (use_graph) PF_HIP(hipGraphLaunch(exec[two ? cur_set_ : 0], st_));
         else iteration(opt);
         int active = 0;
         PF_HIP(hipMemcpyAsync(&active, d_count_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
         PF_HIP(hipStreamSynchronize(st_));
         if (active == 0) { ++it; break; }
+        if (2 * active <= M_) cur_set_ = 1;
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (int s = 0; s < 2; ++s) {
+        if (exec[s]) (void)hipGraphExecDestroy(exec[s]);
+        if (graph[s]) (void)hipGraphDestroy(graph[s]);
+    }
     std::vector<double> E(M_), iters(M_), conv(M_);
     int nfallback = 0;
     d_nfallback_.download(&nfallback, 1, st_);
@@ -1062,7 +1125,10 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
         stats->store_bytes = bytes;
         stats->iterations = it;
         stats->graph = use_graph;
-        stats->digest_waves = tuned_waves_;
+        stats->digest_waves = sets_[0].waves;
+        stats->digest_item_values = sets_[0].values;
+        stats->digest_tail_waves = sets_[1].waves;
+        stats->digest_tail_item_values = sets_[1].values;
         stats->nproblems = P_;
         stats->sp2_fallbacks = nfallback;
     }

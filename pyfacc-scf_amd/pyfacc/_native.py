@@ -238,7 +238,8 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("t_total", ctypes.c_double), ("t_setup", ctypes.c_double), ("t_eri", ctypes.c_double),
                 ("t_scf", ctypes.c_double), ("eri_values", ctypes.c_double), ("store_bytes", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("graph", ctypes.c_int), ("digest_waves", ctypes.c_int),
-                ("nproblems", ctypes.c_int), ("sp2_fallbacks", ctypes.c_int)]
+                ("nproblems", ctypes.c_int), ("sp2_fallbacks", ctypes.c_int), ("digest_tail_waves", ctypes.c_int),
+                ("digest_item_values", ctypes.c_double), ("digest_tail_item_values", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

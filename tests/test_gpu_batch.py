@@ -98,6 +98,31 @@ def test_batch_mixed_fp64_opening_uhf(gpu_lib):
     assert z["converged"].all()
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_autotune_shapes_agree(gpu_lib, tmp_path, monkeypatch, graph):
+    """Every digest shape the autotuner can pick -- 4 or 8 wavefronts per
+    workgroup x work items of 65536, 262144 or 1048576 stored integrals or
+    whole molecules -- in either role (while more than half the molecules
+    iterate, and after) gives the same energies, replayed from graphs or
+    launched eagerly, and a cached choice is read back as such."""
+    import json
+    import pyfacc
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("PYFACC_TUNE_CACHE", str(cache))
+    mols = _instances(IDS)
+    ref = pyfacc.batch_scf(mols, graph=True, **OPTS)  # tunes and writes the cache
+    key = [k for k in json.loads(cache.read_text()) if k.startswith("batch_digest_shapes:")][0]
+    items = [65536.0, 262144.0, 1048576.0, 1e18]
+    for c0, c1 in [(c, (c + 1) % 8) for c in range(8)] + [(5, 5)]:  # every shape in both roles; one in both
+        cache.write_text(json.dumps({key: c0 * 8 + c1}))
+        r = pyfacc.batch_scf(mols, graph=graph, **OPTS)
+        assert r["converged"].all()
+        st = r["stats"]
+        assert (st["digest_waves"], st["digest_item_values"]) == ((4, 8)[c0 // 4], items[c0 % 4])
+        assert (st["digest_tail_waves"], st["digest_tail_item_values"]) == ((4, 8)[c1 // 4], items[c1 % 4])
+        assert np.abs(r["energy"] - ref["energy"]).max() < 1e-10
+
+
 def test_batch_plain_roothaan_and_errors(gpu_lib):
     import pyfacc
     mols = _instances([3, 7, 9])  # three single waters
