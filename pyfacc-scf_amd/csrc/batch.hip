@@ -21,6 +21,7 @@
 #include "bdense.h"
 #include "boys.h"
 #include "cart.h"
+#include "hostpar.h"
 #include "linalg.h"
 
 namespace pf {
@@ -174,6 +175,20 @@ __global__ void __launch_bounds__(256) bdtot_kernel(BatchDev b, const double *D,
         const int m = b.pmol[p];
         if (!b.muhf[m] || p != b.mprob[m]) return;
         Dtot[b.moff[m] + e] = D[b.poff[p] + e] + D[b.poff[p + 1] + e];
+    }
+}
+
+// Superposition of atomic densities: D_p = f x the block-diagonal of the
+// atoms' densities (f = 1 RHF, 1/2 per spin UHF).  Basis function i of
+// molecule m has bfi[3 (mbf[m] + i) ..] = (atom, the row's start in tab, its
+// index on the isolated atom).
+__global__ void __launch_bounds__(256) bsad_kernel(BatchDev b, const int *bfi, const int *mbf, const double *tab,
+                                                   double *D) {
+    PROBLEM_LOOP(b) {
+        const int m = b.pmol[p], r = e / n, c = e - r * n;
+        const int *ir = bfi + 3 * (mbf[m] + r), *ic = bfi + 3 * (mbf[m] + c);
+        const double f = b.muhf[m] ? 0.5 : 1.0;
+        D[b.poff[p] + e] = ir[0] == ic[0] ? f * tab[ir[1] + ic[2]] : 0.0;
     }
 }
 
@@ -575,9 +590,18 @@ void BatchEngine::setup() {
     std::vector<int> shoff(M_ + 1, 0), aoff(M_ + 1, 0);
     std::vector<double> axyz;
     std::vector<int> Z;
+    // each molecule's basis and nuclear repulsion on host threads
+    std::vector<Basis> bases(M_);
+    enuc_.resize(M_);
+    parallel_ranges(M_, host_threads(M_, 128), [&](int, int m0, int m1) {
+        for (int m = m0; m < m1; ++m) {
+            bases[m] = build_basis_blocks(blocks, mols_[m].species, mols_[m].mol, renorm_);
+            enuc_[m] = nuclear_repulsion(mols_[m].mol);
+        }
+    });
     for (int m = 0; m < M_; ++m) {
         const auto &bm = mols_[m];
-        Basis bs = build_basis_blocks(blocks, bm.species, bm.mol, renorm_);
+        const Basis &bs = bases[m];
         for (const auto &sh : bs.shells)
             if (sh.l >= 2) throw Error("batch: molecule " + std::to_string(m) + ": s and p shells only (d shells: pf_scf)");
         for (auto &s : bs.shells) shells_.push_back(s);
@@ -592,7 +616,6 @@ void BatchEngine::setup() {
         mn_.push_back(n);
         muhf_.push_back(uhf);
         mprob_.push_back((int)pn_.size());
-        enuc_.push_back(nuclear_repulsion(bm.mol));
         nmax_ = std::max(nmax_, n);
         for (int s = 0; s <= uhf; ++s) { pn_.push_back(n); pmol_.push_back(m); }
     }
@@ -706,23 +729,42 @@ void BatchEngine::setup() {
     }
     PF_HIP(hipStreamSynchronize(st_));
     phase("pairs pass 1 + Schwarz");
-    // pass 2: per molecule keep Q * Qmax(molecule) >= tau, sorted by Q descending
+    // pass 2: per molecule keep Q * Qmax(molecule) >= tau, sorted by Q
+    // descending; molecules in contiguous ranges on host threads, the ranges'
+    // lists concatenated in molecule order
     std::array<std::vector<double>, 3> Q;
     std::array<std::vector<std::pair<int, int>>, 3> kept;
-    for (int c = 0; c < 3; ++c) pbeg_[c].assign(1, 0);
-    for (int m = 0; m < M_; ++m) {
-        double qmax = 0.0;
-        for (int c = 0; c < 3; ++c)
-            for (int i = beg1[c][m]; i < beg1[c][m + 1]; ++i) qmax = std::max(qmax, Q1[c][i]);
-        for (int c = 0; c < 3; ++c) {
+    {
+        const int T = host_threads(M_, 128);
+        std::vector<std::array<std::vector<double>, 3>> Qt(T);
+        std::vector<std::array<std::vector<std::pair<int, int>>, 3>> kt(T);
+        std::vector<std::array<std::vector<int>, 3>> nt(T);  // pairs kept per molecule
+        parallel_ranges(M_, T, [&](int t, int m0, int m1) {
             std::vector<int> idx;
-            for (int i = beg1[c][m]; i < beg1[c][m + 1]; ++i)
-                if (Q1[c][i] * qmax >= schwarz_) idx.push_back(i);
-            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return Q1[c][x] > Q1[c][y]; });
-            for (int i : idx) { kept[c].push_back(all[c][i]); Q[c].push_back(Q1[c][i]); }
-            pbeg_[c].push_back((int)kept[c].size());
+            for (int m = m0; m < m1; ++m) {
+                double qmax = 0.0;
+                for (int c = 0; c < 3; ++c)
+                    for (int i = beg1[c][m]; i < beg1[c][m + 1]; ++i) qmax = std::max(qmax, Q1[c][i]);
+                for (int c = 0; c < 3; ++c) {
+                    idx.clear();
+                    for (int i = beg1[c][m]; i < beg1[c][m + 1]; ++i)
+                        if (Q1[c][i] * qmax >= schwarz_) idx.push_back(i);
+                    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return Q1[c][x] > Q1[c][y]; });
+                    for (int i : idx) { kt[t][c].push_back(all[c][i]); Qt[t][c].push_back(Q1[c][i]); }
+                    nt[t][c].push_back((int)idx.size());
+                }
+            }
+        });
+        for (int c = 0; c < 3; ++c) {
+            pbeg_[c].assign(1, 0);
+            for (int t = 0; t < T; ++t) {
+                kept[c].insert(kept[c].end(), kt[t][c].begin(), kt[t][c].end());
+                Q[c].insert(Q[c].end(), Qt[t][c].begin(), Qt[t][c].end());
+                for (int k : nt[t][c]) pbeg_[c].push_back(pbeg_[c].back() + k);
+            }
         }
     }
+    phase("pairs pass 2, host selection");
     for (int c = 0; c < 3; ++c) {
         fill_pairset(pairs_[c], shells_, d_shells_.p, kept[c], bound_max, bound_max, false, st_);
         pairs_[c].d_Q.upload(Q[c].data(), Q[c].size(), st_);
@@ -734,37 +776,57 @@ void BatchEngine::setup() {
     const double t0 = now();
     auto &work = work_;  // stored values per bra pair
     auto &brq = brq_;    // per class: bra range of each molecule
+    double t_plan = 0.0;
     int q = 0;
     for (int x = 0; x < 3; ++x)
         for (int y = 0; y <= x; ++y, ++q) {
             const PairSet &B = pairs_[x], &K = pairs_[y];
             const bool same = x == y;
             const int nint = nc(B.la) * nc(B.lb) * nc(K.la) * nc(K.lb);
+            const double tq = now();
             std::vector<long long> off(std::max(B.n, 1), 0);
             work[q].assign(std::max(B.n, 1), 0.0);
             std::vector<int> nk(std::max(B.n, 1), 0), kb(std::max(B.n, 1), 0), br(2 * M_);
-            long long tot = 0;
-            for (int m = 0; m < M_; ++m) {
-                br[2 * m] = pbeg_[x][m];
-                br[2 * m + 1] = pbeg_[x][m + 1];
-                const int k0 = pbeg_[y][m], k1 = pbeg_[y][m + 1];
-                for (int i = pbeg_[x][m]; i < pbeg_[x][m + 1]; ++i) {
-                    const double thr = schwarz_ / Q[x][i];
-                    int lo = k0, hi = k1;  // kets sorted by Q descending: prefix with Q >= thr
-                    while (lo < hi) {
-                        const int mid = (lo + hi) / 2;
-                        if (Q[y][mid] >= thr) lo = mid + 1; else hi = mid;
+            // molecules in contiguous ranges on host threads: offsets within
+            // the range first, then shifted by the ranges before it
+            const int T = host_threads(M_, 128);
+            std::vector<long long> tot_t(T + 1, 0);
+            std::vector<double> vals_t(T, 0.0);
+            parallel_ranges(M_, T, [&](int t, int m0, int m1) {
+                long long tot = 0;
+                for (int m = m0; m < m1; ++m) {
+                    br[2 * m] = pbeg_[x][m];
+                    br[2 * m + 1] = pbeg_[x][m + 1];
+                    const int k0 = pbeg_[y][m], k1 = pbeg_[y][m + 1];
+                    for (int i = pbeg_[x][m]; i < pbeg_[x][m + 1]; ++i) {
+                        const double thr = schwarz_ / Q[x][i];
+                        int lo = k0, hi = k1;  // kets sorted by Q descending: prefix with Q >= thr
+                        while (lo < hi) {
+                            const int mid = (lo + hi) / 2;
+                            if (Q[y][mid] >= thr) lo = mid + 1; else hi = mid;
+                        }
+                        int n = lo - k0;
+                        if (same) n = std::min(n, i - k0 + 1);
+                        nk[i] = n;
+                        kb[i] = k0;
+                        off[i] = tot;
+                        tot += (long long)round64(n) * nint;
+                        vals_t[t] += (double)n * nint;
+                        work[q][i] = (double)round64(n) * nint;
                     }
-                    int n = lo - k0;
-                    if (same) n = std::min(n, i - k0 + 1);
-                    nk[i] = n;
-                    kb[i] = k0;
-                    off[i] = tot;
-                    tot += (long long)round64(n) * nint;
-                    eri_values_ += (double)n * nint;
-                    work[q][i] = (double)round64(n) * nint;
                 }
+                tot_t[t + 1] = tot;
+            });
+            for (int t = 0; t < T; ++t) {
+                tot_t[t + 1] += tot_t[t];
+                eri_values_ += vals_t[t];
             }
+            parallel_ranges(M_, T, [&](int t, int m0, int m1) {
+                if (m0 == m1 || t == 0) return;
+                for (int i = pbeg_[x][m0]; i < pbeg_[x][m1]; ++i) off[i] += tot_t[t];
+            });
+            const long long tot = tot_t[T];
+            t_plan += now() - tq;
             s64_[q].alloc(std::max(tot, 1LL));
             s32_[q].alloc(std::max(tot, 1LL));
             soff_[q].upload(off.data(), off.size(), st_);
@@ -789,6 +851,7 @@ void BatchEngine::setup() {
         }
     PF_HIP(hipStreamSynchronize(st_));
     t_eri_ = now() - t0;
+    if (verbose_) std::fprintf(stderr, "batch setup: of which host quartet plan  %8.3f s\n", t_plan);
     for (int s = 0; s < 2; ++s) {  // the autotuner may rebuild them with another shape
         build_items(s, 262144.0);
         sets_[s].waves = 8;
@@ -980,22 +1043,58 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_muse_.p);
     for (auto *b : {&d_E_, &d_Eold_, &d_iters_, &d_conv_}) dev_fill(M_, 0.0, b->p, st_);
     if (opt.guess == PF_GUESS_SAD) {
-        // superposition of atomic densities (as Engine::scf): RHF the sum, UHF half per spin
-        std::vector<double> hD((size_t)ptot_, 0.0);
+        // superposition of atomic densities (as Engine::scf and sad_density):
+        // RHF the sum, UHF half per spin; each element's atom and its place
+        // on the isolated atom are tabulated here and the blocks laid out on
+        // the device
+        std::map<std::string, std::pair<int, int>> spec;  // element -> (offset in tab, nbf of the atom)
+        std::vector<double> tab;
+        std::vector<int> mbf(M_ + 1, 0), bfi;
         for (int m = 0; m < M_; ++m) {
-            const int n = mn_[m], p0 = mprob_[m];
-            const std::vector<double> Dm = sad_density(shells_.data() + shoff_[m], shoff_[m + 1] - shoff_[m], n,
-                                                       mols_[m].species, mols_[m].mol.Z, basis_path_, renorm_);
-            const double f = muhf_[m] ? 0.5 : 1.0;
-            for (int s = 0; s <= muhf_[m]; ++s)
-                for (size_t e = 0; e < (size_t)n * n; ++e) hD[poff_[p0 + s] + e] = f * Dm[e];
+            const auto &bm = mols_[m];
+            const int na = bm.mol.natoms, s0 = shoff_[m], s1 = shoff_[m + 1];
+            std::vector<int> loc(na, 0);
+            mbf[m + 1] = mbf[m] + mn_[m];
+            bfi.resize(3 * (size_t)mbf[m + 1]);
+            for (int s = s0; s < s1; ++s) {  // an atom's shells in the order they have on the isolated atom
+                const Shell &sh = shells_[s];
+                const int a = sh.atom;
+                auto it = spec.find(bm.species[a]);
+                if (it == spec.end()) {
+                    const std::vector<double> Da = atomic_density(basis_path_, renorm_, bm.species[a], bm.mol.Z[a]);
+                    const int nat = (int)std::lround(std::sqrt((double)Da.size()));
+                    it = spec.emplace(bm.species[a], std::make_pair((int)tab.size(), nat)).first;
+                    tab.insert(tab.end(), Da.begin(), Da.end());
+                }
+                const auto [toff, nat] = it->second;
+                for (int k = 0; k < ncart(sh.l); ++k, ++loc[a]) {
+                    if (loc[a] >= nat) throw Error("SAD guess: atomic basis mismatch for " + bm.species[a]);
+                    int *f = &bfi[3 * (size_t)(mbf[m] + sh.bf + k)];
+                    f[0] = a;
+                    f[1] = toff + loc[a] * nat;
+                    f[2] = loc[a];
+                }
+            }
+            for (int a = 0; a < na; ++a)
+                if (loc[a] != spec.at(bm.species[a]).second)
+                    throw Error("SAD guess: atomic basis mismatch for " + bm.species[a]);
         }
-        D_.upload(hD.data(), hD.size(), st_);
+        DevBuf<int> d_bfi, d_mbf;
+        DevBuf<double> d_tab;
+        d_bfi.upload(bfi.data(), bfi.size(), st_);
+        d_mbf.upload(mbf.data(), mbf.size(), st_);
+        d_tab.upload(tab.data(), tab.size(), st_);
+        bsad_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, d_bfi.p, d_mbf.p, d_tab.p, D_.p);
+        PF_HIP(hipStreamSynchronize(st_));  // before the tables go
         bdtot_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, D_.p, Dtot_.p);
     } else {
         // core guess: D from H (rscf.f90:113-114; uscf.f90:140-144)
         bcopy_kernel<<<pgrid(P_, nmax_), 256, 0, st_>>>(dev_, H_.p, d_pmoff_.p, Fx_.p);
         density_step(Fx_.p);
+    }
+    if (verbose_) {
+        PF_HIP(hipStreamSynchronize(st_));
+        std::fprintf(stderr, "batch start: state and guess density     %8.3f s\n", now() - t1);
     }
     // autotune the digest's two shapes -- wavefronts per workgroup x stored
     // integrals per work item -- on the real batch; cached per device and
@@ -1087,6 +1186,7 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
             PF_HIP(hipGraphInstantiate(&exec[s], graph[s], nullptr, nullptr, 0));
         }
     cur_set_ = 0;
+    if (verbose_) std::fprintf(stderr, "batch start: tuning and graph capture   %8.3f s\n", now() - t2);
     int it = 0;
     for (; it < opt.max_iter; ++it) {
         if (use_graph) PF_HIP(hipGraphLaunch(exec[two ? cur_set_ : 0], st_));

@@ -251,13 +251,23 @@ def batch_scf(molecules, basis="sto-3g", graph=True, options=None, **kw):
     molecules: sequence of (species, coords_bohr (natoms, 3), nalpha, nbeta).
     Returns a dict with per-molecule ``energy``, ``iterations``, ``converged``
     arrays and the call's statistics under ``stats``."""
+    from .helper_functions import periodic_table
     opt = options if options is not None else default_options(**kw)
-    mols = [_Mol(sp, xyz) for sp, xyz, _, _ in molecules]
-    nmol = len(mols)
-    natoms = np.array([m.n for m in mols], dtype=np.int32)
-    species = b"".join(m.sp for m in mols)
-    z = np.ascontiguousarray(np.concatenate([m.z for m in mols]).astype(np.int32))
-    xyz = np.ascontiguousarray(np.concatenate([m.xyz for m in mols]).astype(np.float64))
+    nmol = len(molecules)
+    natoms = np.empty(nmol, dtype=np.int32)
+    labels, coords = [], []
+    for i, (sp, c, _, _) in enumerate(molecules):
+        natoms[i] = len(sp)
+        labels.extend(str(s) for s in sp)
+        c = np.asarray(c, dtype=np.float64)
+        if c.shape != (len(sp), 3):
+            raise ValueError(f"molecule {i}: coords must have shape ({len(sp)}, 3), got {c.shape}")
+        coords.append(c)
+    if any(len(s) > 2 for s in set(labels)):
+        raise ValueError("element symbols have at most two characters")
+    species = np.array(labels, dtype="S2").tobytes()  # two bytes per atom, NUL-padded
+    z = np.array([periodic_table[s] for s in labels], dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate(coords)) if coords else np.zeros((0, 3))
     na = np.array([int(t[2]) for t in molecules], dtype=np.int32)
     nb = np.array([int(t[3]) for t in molecules], dtype=np.int32)
     e = np.zeros(nmol)
