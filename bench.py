@@ -8,7 +8,8 @@ threshold 1e-10, DIIS(8), convergence |dE| < 1e-8 and ||dD||_F < 1e-8.
 
 A step is one complete mixed-precision SCF (FP32 incremental J/K on the
 difference density with FP64 Fock accumulation, one full FP64 rebuild inside
-the FP32 phase, then the FP64 switch and finish), started from a
+the FP32 phase, then the switch's full FP64 build and FP32 increments to
+convergence under the FP64 stopping rule), started from a
 superposition of atomic densities; the same SCF from the reference's
 core-Hamiltonian guess is timed beside it.  The same SCF in plain FP64 (full FP64 Fock build every
 iteration) is timed the same way for the comparison.  Inputs are built in HBM

@@ -54,6 +54,7 @@ enum { PF_ERI_DIRECT = 0, PF_ERI_STORED = 1 };
 enum { PF_DIAG_SP2 = 0, PF_DIAG_JACOBI = 1 };
 enum { PF_GUESS_CORE = 0, PF_GUESS_SAD = 1 };
 enum { PF_DIIS_AO = 0, PF_DIIS_ORTHO = 1 };
+enum { PF_FINISH_FP32 = 0, PF_FINISH_FP64 = 1 };
 
 typedef struct pf_options {
     double eps;           /* convergence threshold on |dE| and ||dD||_F (rscf.f90:152-155) */
@@ -95,6 +96,10 @@ typedef struct pf_options {
                              energies and densities; s and p shells, direct builds only (pf_scf,
                              pf_scf_gradient's SCF; not pf_batch_scf).  0: FP64 atomics (repeated runs differ
                              by ~1e-11) */
+    int mixed_finish;     /* mixed precision, after the switch's full FP64 build: PF_FINISH_FP32, the increments
+                             that follow run in FP32 (screened at schwarz_inc; their rounding scales with
+                             ||dD||_F < mixed_switch), accumulated in FP64; PF_FINISH_FP64, they run in FP64.
+                             pf_scf and pf_scf_gradient (pf_batch_scf: FP64 after the switch) */
 } pf_options;
 
 typedef struct pf_result {
@@ -120,7 +125,7 @@ typedef struct pf_result {
 /* Defaults: eps 1e-8, max_iter 100, FP64, full builds, direct, DIIS 8, SP2,
  * renormalised, Schwarz 1e-10, mixed_switch 1e-5, schwarz_inc 1e-12,
  * rebuild_every 0, schwarz_fp32 1e-6, core guess, no level shift (level_shift_until 0.1), AO DIIS error,
- * mixed_anchor 0.1, profile 0, jk_fallback 0, deterministic 0. */
+ * mixed_anchor 0.1, profile 0, jk_fallback 0, deterministic 0, mixed_finish PF_FINISH_FP32. */
 void pf_default_options(pf_options *opt);
 
 /* Message for the last failed call on this thread ("" if none). */

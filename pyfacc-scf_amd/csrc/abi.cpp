@@ -112,6 +112,7 @@ void pf_default_options(pf_options *o) {
     o->profile = 0;
     o->jk_fallback = 0;
     o->deterministic = 0;
+    o->mixed_finish = PF_FINISH_FP32;
 }
 
 const char *pf_last_error(void) { return g_last_error.c_str(); }
@@ -207,6 +208,7 @@ int pf_scf(int natoms, const char *species, const int32_t *z, const double *coor
         if (o.max_iter < 1) throw Error("max_iter must be >= 1");
         if (!(o.level_shift >= 0.0)) throw Error("level_shift must be >= 0");
         if (o.diis_basis != PF_DIIS_AO && o.diis_basis != PF_DIIS_ORTHO) throw Error("diis_basis must be 0 or 1");
+        if (o.mixed_finish != PF_FINISH_FP32 && o.mixed_finish != PF_FINISH_FP64) throw Error("mixed_finish must be 0 or 1");
         Input in = make_input(natoms, species, z, coords);
         Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,
                    o.schwarz > 0 ? o.schwarz : 1e-14);
@@ -273,6 +275,7 @@ int pf_scf_gradient(int natoms, const char *species, const int32_t *z, const dou
         if (o.max_iter < 1) throw Error("max_iter must be >= 1");
         if (!(o.level_shift >= 0.0)) throw Error("level_shift must be >= 0");
         if (o.diis_basis != PF_DIIS_AO && o.diis_basis != PF_DIIS_ORTHO) throw Error("diis_basis must be 0 or 1");
+        if (o.mixed_finish != PF_FINISH_FP32 && o.mixed_finish != PF_FINISH_FP64) throw Error("mixed_finish must be 0 or 1");
         if (!grad) throw Error("null gradient output");
         Input in = make_input(natoms, species, z, coords);
         Engine eng(in.mol, in.species, resolve_basis_path(basis_name ? basis_name : ""), o.renormalize != 0,

@@ -862,6 +862,11 @@ void Engine::scf(int nalpha, int nbeta, const pf_options &opt, pf_result &res, d
                 // before the anchor rebuild (which replaces G) screening may be looser
                 build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, anchored ? 0.0 : opt.schwarz_fp32);
                 ++res.fock_builds_fp32;
+            } else if (opt.precision == PF_PREC_MIXED && opt.mixed_finish == PF_FINISH_FP32) {
+                // after the switch's full FP64 build: the increments' FP32 rounding
+                // is ~1e-7 of G[dD] with ||dD||_F < mixed_switch, far below eps
+                build_g<float>(d0, d1, G0, G1, true, MODE_DIRECT, opt.schwarz_inc);
+                ++res.fock_builds_fp32;
             } else {
                 build_g<double>(d0, d1, G0, G1, true, gmode, opt.schwarz_inc);
                 ++res.fock_builds_fp64;

@@ -24,7 +24,8 @@ class Options(ctypes.Structure):
                 ("rebuild_every", ctypes.c_int), ("schwarz_fp32", ctypes.c_double), ("guess", ctypes.c_int),
                 ("level_shift", ctypes.c_double), ("level_shift_until", ctypes.c_double),
                 ("diis_basis", ctypes.c_int), ("mixed_anchor", ctypes.c_double), ("profile", ctypes.c_int),
-                ("jk_fallback", ctypes.c_int), ("deterministic", ctypes.c_int)]
+                ("jk_fallback", ctypes.c_int), ("deterministic", ctypes.c_int),
+                ("mixed_finish", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):

@@ -161,7 +161,7 @@ def test_options_layout_matches_the_library():
     expect = dict(eps=1e-8, max_iter=100, precision=0, incremental=0, eri_mode=0, diis=8, diag=0, renormalize=1,
                   schwarz=1e-10, mixed_switch=1e-5, verbose=0, schwarz_inc=1e-12, rebuild_every=0,
                   schwarz_fp32=1e-6, guess=0, level_shift=0.0, level_shift_until=0.1, diis_basis=0,
-                  mixed_anchor=0.1, profile=0, jk_fallback=0, deterministic=0)
+                  mixed_anchor=0.1, profile=0, jk_fallback=0, deterministic=0, mixed_finish=0)
     assert [f for f, _ in pyfacc._native.Options._fields_] == list(expect)
     for k, v in expect.items():
         assert getattr(o, k) == v, k

@@ -45,6 +45,23 @@ def test_h2o64_mixed_and_fp64_at_bench_settings(gpu_lib):
     assert np.abs(out[1]["D"] - out[0]["D"]).max() < 1e-5
 
 
+def test_h2o64_mixed_finish_forms_agree(gpu_lib):
+    """After the switch's full FP64 build the mixed SCF's remaining increments
+    run in FP32 (mixed_finish 0, the default) or FP64 (1).  The FP32 rounding
+    of an increment scales with ||dD||_F < mixed_switch, so both end on the
+    same energy (measured on MI355X: 16 iterations each, equal to 1e-11 Eh)
+    and the FP32 form makes only the anchor and switch builds in FP64."""
+    import pyfacc
+    sp, xyz = load_xyz("h2o_64.xyz")
+    r32 = pyfacc.scf(sp, xyz, 320, 320, precision=1, mixed_finish=0, **BENCH)
+    r64 = pyfacc.scf(sp, xyz, 320, 320, precision=1, mixed_finish=1, **BENCH)
+    assert r32["converged"] and r64["converged"]
+    assert r32["fock_builds_fp64"] == 2 and r64["fock_builds_fp64"] > 2
+    print(f"(H2O)64 mixed finish: E_fp32 - E_fp64 = {r32['energy'] - r64['energy']:.2e} Eh, iterations "
+          f"{r32['iterations']} / {r64['iterations']}")
+    assert abs(r32["energy"] - r64["energy"]) < 1e-8
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_h2o16_production_screening_energy_bound(gpu_lib, fx, prec):
     """(H2O)16 STO-3G at the default screening (Q_ab Q_cd max|D| < 1e-10, and
