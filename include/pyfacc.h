@@ -82,8 +82,8 @@ typedef struct pf_options {
                              with X the orthogonaliser (pf_scf and pf_scf_gradient; pf_batch_scf: AO only) */
     double mixed_anchor;  /* mixed precision: the full FP64 rebuild inside the FP32 phase comes when ||dD||_F
                              first falls below mixed_anchor (0: none; pf_scf and pf_scf_gradient); pf_batch_scf:
-                             FP64 builds until ||dD||_F first falls below mixed_anchor, FP32 after (0: FP32 from
-                             the start) */
+                             full FP64 builds until ||dD||_F first falls below mixed_anchor, FP32 increments after
+                             (0: FP32 increments from the start) */
     int profile;          /* 1: time each J/K class kernel with HIP events around it (pf_scf_profile's
                              milliseconds; the events add short gaps to the build), 0: no events */
     int jk_fallback;      /* testing: direct J/K builds take the paths that large systems need, at any
@@ -98,8 +98,8 @@ typedef struct pf_options {
                              by ~1e-11) */
     int mixed_finish;     /* mixed precision, after the switch's full FP64 build: PF_FINISH_FP32, the increments
                              that follow run in FP32 (screened at schwarz_inc; their rounding scales with
-                             ||dD||_F < mixed_switch), accumulated in FP64; PF_FINISH_FP64, they run in FP64.
-                             pf_scf and pf_scf_gradient (pf_batch_scf: FP64 after the switch) */
+                             ||dD||_F < mixed_switch), accumulated in FP64; PF_FINISH_FP64, they run in FP64
+                             (pf_batch_scf: full FP64 builds).  pf_scf, pf_scf_gradient and pf_batch_scf */
 } pf_options;
 
 typedef struct pf_result {

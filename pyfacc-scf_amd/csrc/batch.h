@@ -38,6 +38,12 @@ struct BatchClass {
     const float *v32 = nullptr;
     const int *brange = nullptr;  // per molecule: bra pairs brange[2m] .. brange[2m+1]-1
     int same = 0;
+    // FP32 increments: Schwarz bounds of the bra and ket pairs (kets of a
+    // molecule sorted by Q descending), each bra's molecule, and per bra the
+    // ket count that passes Q_i Q_j max|dD| >= tau this iteration
+    const double *bQ = nullptr, *kQ = nullptr;
+    const int *bmol = nullptr;
+    int *nkeff = nullptr;
 };
 
 // A digest work item: molecule m's bra pairs from (class q0, pair i0) to
@@ -58,7 +64,7 @@ struct BatchDev {
     const long long *pdiis = nullptr;                             // DIIS ring offsets
     const long long *psp2 = nullptr;  // Gershgorin scratch offsets (orthogonaliser's molecule view)
     int *mactive = nullptr;   // 1 while the molecule iterates
-    int *mprec = nullptr;     // 1 = FP64 integrals; mixed form: 2 = FP64 opening, 0 = FP32 phase, then 1
+    int *mprec = nullptr;     // 1 = full FP64 builds; mixed form: 2 = FP64 opening, 0 = FP32 increments, then 1
     int *mhead = nullptr, *mnstore = nullptr, *muse = nullptr;  // DIIS ring head / size / extrapolate flag
     double *mcoef = nullptr;  // DIIS coefficients [M][ndiis]
     double *mB = nullptr;     // DIIS B matrices [M][ndiis][ndiis]
@@ -122,6 +128,16 @@ class BatchEngine {
     DevBuf<double> S_, H_, X_, Dtot_;                    // molecule pools
     DevBuf<double> D_, Dold_, F_, Fx_, Fp_, W1_, W2_, Pr_;  // problem pools
     DevBuf<double> dF_, dE_;                             // DIIS rings
+    // mixed form: the two-electron part G (F = H + G) and the density it was
+    // built from (problem pools), the FP32 difference densities of the
+    // increments (problem pool; UHF molecules' spin sum in a molecule pool) and
+    // each molecule's max|dD|
+    bool mixed_ = false;
+    double schwarz_inc_ = 1e-12;  // ket cut of the increments after the switch (Q Q max|dD|)
+    DevBuf<double> G_, Db_;
+    DevBuf<float> dD32_, dDt32_, dmax_;
+    std::array<DevBuf<int>, 3> d_pairmol_;  // per pair class: each pair's molecule
+    std::array<DevBuf<int>, 6> nkeff_;      // per quartet class: kets of each bra this iteration
     std::array<PairSet, 3> pairs_;
     std::array<std::vector<int>, 3> pbeg_;  // per class: molecule m's pairs pbeg[m] .. pbeg[m+1]-1
     // stored integrals per quartet class

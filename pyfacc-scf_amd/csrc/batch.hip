@@ -28,9 +28,13 @@ namespace pf {
 
 size_t batch_digest_lds(int nmax);
 void batch_digest_prepare(int nmax);
+// Full FP64 builds of the molecules with mprec != 0; with G (mixed form) they
+// also record G and the density built from, and the molecules with mprec == 0
+// get FP32 increments on dD = D - Db (kets cut at Q Q max|dD| < tau).
 void launch_batch_digest(const BatchDev &b, const std::array<BatchClass, 6> &cls, const BatchItem *items,
                          int nitems, const int *msplit, const double *D, const double *Dtot, const double *H,
-                         double *F, double *gacc, int waves, hipStream_t st);
+                         double *F, double *gacc, int waves, hipStream_t st, double *G, double *Db, float *dD32,
+                         float *dDt32, float *dmax, double tau, double tau_inc, const std::array<int, 6> &nbra);
 
 namespace {
 
@@ -436,15 +440,17 @@ __global__ void __launch_bounds__(256) bgersh_bounds_kernel(BatchDev b, double *
 }
 
 // Energy, convergence and freezing, one workgroup per molecule (rscf.f90:142-156
-// stopping rule: |dE| < eps and ||dD||_F < eps).  Mixed form: integrals switch
-// from FP32 to FP64 when ||dD||_F < mixed_switch (DIIS restarts); only the FP64
-// phase may converge.
+// stopping rule: |dE| < eps and ||dD||_F < eps).  Mixed form (mprec): FP64
+// opening (2) until ||dD||_F < mixed_open, FP32 increments (0) until
+// ||dD||_F < mixed_switch, then one full FP64 build (1) and, with finish32,
+// FP32 increments again (3), else full FP64 builds; only the phase after the
+// switch may converge.  restart: DIIS starts a new subspace at the switch.
 struct ConvArgs {
     const double *D, *Dold, *F, *H, *enuc;
     double *E, *Eold, *iters, *conv;
     int *count;
     double eps, mixed_switch, mixed_open;
-    int max_iter;
+    int max_iter, finish32, restart;
 };
 __global__ void __launch_bounds__(256) benergy_kernel(BatchDev b, ConvArgs a) {
     __shared__ double sh[4];
@@ -473,10 +479,15 @@ __global__ void __launch_bounds__(256) benergy_kernel(BatchDev b, ConvArgs a) {
         bool active = true;
         if (b.mprec[m] == 2 && errD < a.mixed_open) b.mprec[m] = 0;  // FP64 opening done: FP32 phase
         if (b.mprec[m] == 0) {
-            if (errD < a.mixed_switch) { b.mprec[m] = 1; b.mnstore[m] = 0; b.mhead[m] = 0; }
+            if (errD < a.mixed_switch) {
+                b.mprec[m] = 1;
+                if (a.restart) { b.mnstore[m] = 0; b.mhead[m] = 0; }
+            }
         } else if (dE < a.eps && errD < a.eps) {
             a.conv[m] = 1.0;
             active = false;
+        } else if (b.mprec[m] == 1 && a.finish32) {
+            b.mprec[m] = 3;  // after the switch's full FP64 build
         }
         if (it >= a.max_iter) active = false;
         if (!active) b.mactive[m] = 0;
@@ -848,7 +859,20 @@ void BatchEngine::setup() {
             bc.ksa = K.d_sa.p; bc.ksb = K.d_sb.p; bc.koa = K.d_oa.p; bc.kob = K.d_ob.p;
             bc.off = soff_[q].p; bc.nket = snket_[q].p; bc.kbase = skbase_[q].p;
             bc.v64 = s64_[q].p; bc.v32 = s32_[q].p; bc.brange = sbr_[q].p; bc.same = same;
+            bc.bQ = B.d_Q.p; bc.kQ = K.d_Q.p;
+            nkeff_[q].alloc(std::max(B.n, 1));
+            bc.nkeff = nkeff_[q].p;
         }
+    // each pair's molecule, per pair class (the FP32 increments' ket cut)
+    for (int c = 0; c < 3; ++c) {
+        std::vector<int> pm(std::max(pairs_[c].n, 1), 0);
+        for (int m = 0; m < M_; ++m)
+            for (int i = pbeg_[c][m]; i < pbeg_[c][m + 1]; ++i) pm[i] = m;
+        d_pairmol_[c].upload(pm.data(), pm.size(), st_);
+    }
+    q = 0;
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y <= x; ++y, ++q) cls_[q].bmol = d_pairmol_[x].p;
     PF_HIP(hipStreamSynchronize(st_));
     t_eri_ = now() - t0;
     if (verbose_) std::fprintf(stderr, "batch setup: of which host quartet plan  %8.3f s\n", t_plan);
@@ -999,7 +1023,12 @@ void BatchEngine::build_items(int set, double item_values) {
 
 void BatchEngine::digest(int set) {
     const ItemSet &S = sets_[set];
-    launch_batch_digest(dev_, cls_, S.items.p, S.n, S.msplit.p, D_.p, Dtot_.p, H_.p, F_.p, gacc_.p, S.waves, st_);
+    std::array<int, 6> nbra{};
+    int q = 0;
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y <= x; ++y, ++q) nbra[q] = pairs_[x].n;
+    launch_batch_digest(dev_, cls_, S.items.p, S.n, S.msplit.p, D_.p, Dtot_.p, H_.p, F_.p, gacc_.p, S.waves, st_,
+                        mixed_ ? G_.p : nullptr, Db_.p, dD32_.p, dDt32_.p, dmax_.p, schwarz_, schwarz_inc_, nbra);
 }
 
 // One SCF iteration for every active molecule (no host decision inside).
@@ -1022,7 +1051,9 @@ void BatchEngine::iteration(const pf_options &opt) {
         density_step(F_.p);
     }
     ConvArgs c{D_.p, Dold_.p, F_.p, H_.p, d_enuc_.p, d_E_.p, d_Eold_.p, d_iters_.p, d_conv_.p, d_count_.p,
-               opt.eps, opt.mixed_switch, opt.mixed_anchor, opt.max_iter};
+               opt.eps, opt.mixed_switch, opt.mixed_anchor, opt.max_iter,
+               opt.precision == PF_PREC_MIXED && opt.mixed_finish == PF_FINISH_FP32 ? 1 : 0,
+               1};
     benergy_kernel<<<M_, 256, 0, st_>>>(dev_, c);
 }
 
@@ -1037,6 +1068,15 @@ void BatchEngine::run(const pf_options &opt, bool use_graph, double *energy, int
     // state: frozen flags, precision, DIIS rings, energies
     // mixed form: FP64 opening (2) until ||dD||_F < mixed_anchor (none if 0), FP32 (0), FP64 (1)
     const int prec0 = opt.precision != PF_PREC_MIXED ? 1 : (opt.mixed_anchor > 0.0 ? 2 : 0);
+    mixed_ = opt.precision == PF_PREC_MIXED;
+    schwarz_inc_ = opt.schwarz_inc > 0.0 ? opt.schwarz_inc : schwarz_;
+    if (mixed_) {
+        // G and the built density start at zero: an FP32 phase from the first
+        // iteration (mixed_anchor 0) builds its first increment on all of D
+        G_.alloc(ptot_); Db_.alloc(ptot_); dD32_.alloc(ptot_); dDt32_.alloc(mtot_); dmax_.alloc(M_);
+        dev_fill(ptot_, 0.0, G_.p, st_);
+        dev_fill(ptot_, 0.0, Db_.p, st_);
+    }
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, prec0, d_mprec_.p);
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mhead_.p);
     fill_int_kernel<<<(M_ + 255) / 256, 256, 0, st_>>>(M_, 0, d_mnstore_.p);

@@ -79,6 +79,33 @@ def test_batch_mixed_within_bar(gpu_lib, batch64):
     assert np.abs(m["energy"] - r["energy"]).max() < 1e-6
 
 
+@pytest.mark.parametrize("finish", [0, 1])
+def test_batch_mixed_increments_split_items(gpu_lib, batch64, tmp_path, monkeypatch, finish):
+    """The mixed form's FP32 increments (J/K of dD = D - D_built added to an
+    FP64 G, kets cut at Q Q max|dD| < tau), after the switch too (finish 0)
+    or full FP64 builds there (finish 1), with whole molecules per work item
+    and with molecules split over items of 65536 stored integrals (their
+    increments summed through the device accumulators): every form ends
+    within the 1e-6 Eh bar of FP64, and the two item shapes within 1e-8."""
+    import json
+    import pyfacc
+    mols, r = batch64
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("PYFACC_TUNE_CACHE", str(cache))
+    pyfacc.batch_scf(mols, graph=True, **OPTS)  # tunes and writes the cache
+    key = [k for k in json.loads(cache.read_text()) if k.startswith("batch_digest_shapes:")][0]
+    out = {}
+    for code in (7 * 8 + 7, 4 * 8 + 4):  # 8 waves x whole molecules; 8 waves x 65536-value items
+        cache.write_text(json.dumps({key: code}))
+        m = pyfacc.batch_scf(mols, graph=True, precision=1, mixed_finish=finish, **OPTS)
+        assert m["converged"].all()
+        assert np.abs(m["energy"] - r["energy"]).max() < 1e-6
+        out[code] = m
+        st = m["stats"]
+        assert st["digest_item_values"] == (1e18 if code == 63 else 65536.0)
+    assert np.abs(out[63]["energy"] - out[36]["energy"]).max() < 1e-8
+
+
 def test_batch_mixed_fp64_opening_uhf(gpu_lib):
     """OH-radical clusters whose FP32-from-the-start UHF iterations land on a
     different stationary point than FP64 (3.9e-4 Eh apart in the 4096-instance
