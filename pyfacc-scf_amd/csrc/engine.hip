@@ -417,6 +417,32 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
     hipEvent_t *ev = pb.ev;
     if (profile_classes)
         for (int e = 0; e < 12; ++e) ev[e] = event_acquire();
+    // direct builds: every class's work plan in two launches before the
+    // direct kernels, the J'_cd sums in two after them
+    const bool planned = mode == MODE_DIRECT;
+    PlanArgs<R> plan;
+    if (planned) {
+        plan.n = (int)classes_.size();
+        plan.tau = a.tau;
+        plan.dmax = a.dmax;
+        plan.J = J;
+        plan.nbf = n;
+        plan.fx = a.fx;
+        for (size_t q = 0; q < classes_.size(); ++q) {
+            const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
+            PlanClass<R> &c = plan.c[q];
+            if constexpr (std::is_same<R, double>::value) { c.bra = B.view64(); c.ket = K.view64(); }
+            else { c.bra = B.view32(); c.ket = K.view32(); }
+            c.same = classes_[q].bra == classes_[q].ket;
+            c.chunk = direct_chunk<R>(B.la, B.lb, K.la, K.lb);
+            c.ncd = ncart(K.la) * ncart(K.lb);
+            c.nket = plan_nket_[q].p; c.item_off = plan_off_[q].p; c.work = plan_work_[q].p;
+            c.jhi = plan_jhi_[q].p; c.joff = plan_joff_[q].p;
+            c.jbuf = static_cast<const double *>(jbuf_[q].p);
+            c.jcap = c.jbuf ? ((jk_fallback & 2) ? 0 : jcap_[q]) : -1;
+        }
+        launch_plan<R>(plan, st_);
+    }
     for (size_t q = 0; q < classes_.size(); ++q) {
         const PairSet &B = pairs_[classes_[q].bra], &K = pairs_[classes_[q].ket];
         if constexpr (std::is_same<R, double>::value) { a.bra = B.view64(); a.ket = K.view64(); }
@@ -433,10 +459,12 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
         a.joff = plan_joff_[q].p; a.jbuf = static_cast<double *>(jbuf_[q].p); a.jhi = plan_jhi_[q].p;
         a.jcap = (jk_fallback & 2) ? 0 : jcap_[q];  // 0: no ket list fits, J'_cd through device atomics
         a.katomic = (jk_fallback & 1) ? 1 : 0;
+        a.planned = planned ? 1 : 0;
         a.ev0 = profile_classes ? ev[2 * q] : nullptr;
         a.ev1 = profile_classes ? ev[2 * q + 1] : nullptr;
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
     }
+    if (planned) launch_jcd_sums<R>(plan, st_);
     if (ngen_ > 0) {
         // quartets with a d shell, in FP64 in either precision (FP64 densities)
         const double *Dj = D0;

@@ -675,12 +675,14 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     a.krows = sh.krows && !args.katomic;
     const size_t dyn = a.krows ? sh.dyn : sizeof(double) * NA * NB;
     const int grid = sh.grid;
-    jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
-    jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);
+    if (!a.planned) {
+        jk_count_kernel<R><<<(a.bra.n + 255) / 256, 256, 0, st>>>(a);
+        jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);
+    }
     if (a.ev0) PF_HIP(hipEventRecord(a.ev0, st));
     kern<<<grid, 64 * W, dyn, st>>>(a, a.bra.prim, a.item_off);
     if (a.ev1) PF_HIP(hipEventRecord(a.ev1, st));
-    if (a.jbuf) {
+    if (a.jbuf && !a.planned) {
         constexpr int NCD = nc(LC) * nc(LD);
         const long long nt = (long long)a.ket.n * NCD;
         jk_jrange_kernel<R><<<(a.ket.n + 255) / 256, 256, 0, st>>>(a, a.jhi);

@@ -82,6 +82,7 @@ struct JKArgs {
     long long jcap = 0;
     int krows = 0;  // direct mode: bra exchange rows summed in LDS
     int katomic = 0;  // direct mode: force exchange rows to device atomics (the no-LDS-rows path)
+    int planned = 0;  // direct mode: the work plan and the J'_cd sums are launched for all classes at once
     // direct mode, deterministic accumulation: device scalar 2^s.  Every
     // addition into J', K' (LDS rows and device memory) is then the integer
     // add of round(v 2^s) to the accumulator's 8 bytes read as a two's-
@@ -94,6 +95,42 @@ struct JKArgs {
 };
 
 enum JKMode { MODE_DIRECT = 0, MODE_STORE = 1, MODE_LOAD = 2, MODE_PACKED = 3 };
+
+// Direct-mode work plan of one Fock build for all its quartet classes at once:
+// one launch finds every class's ket prefixes (Q_i Q_j dmax >= tau) and item
+// counts, one scans them (a workgroup per class); after the direct kernels
+// one launch finds, per ket, the bras whose prefix holds it, and one sums the
+// J'_cd buffer's columns into J'.  Six launches per build instead of 24.
+template <typename R>
+struct PlanClass {
+    PairView<R> bra, ket;
+    int same = 0, chunk = 0, ncd = 0;  // ncd: components of a ket pair (J'_cd values per slot)
+    int *nket = nullptr, *item_off = nullptr, *work = nullptr, *jhi = nullptr;
+    long long *joff = nullptr;
+    long long jcap = -1;               // J'_cd buffer capacity in slots (-1: no buffer)
+    const double *jbuf = nullptr;
+    int cb0 = 0, cb1 = 0;              // this class's blocks of the count launch
+    int rb0 = 0, rb1 = 0;              // ... of the J'_cd range launch
+    int sb0 = 0, sb1 = 0;              // ... of the J'_cd sum launch (x dimension)
+};
+template <typename R>
+struct PlanArgs {
+    PlanClass<R> c[6];
+    int n = 0;                         // classes
+    R tau = 0;
+    const R *dmax = nullptr;
+    double *J = nullptr;
+    int nbf = 0;
+    const double *fx = nullptr;        // deterministic accumulation scale (nullptr: FP64 atomics)
+};
+// plan (count + scan) before the direct kernels; J'_cd ranges and sums after them
+template <typename R>
+void launch_plan(PlanArgs<R> &p, hipStream_t st);
+template <typename R>
+void launch_jcd_sums(PlanArgs<R> &p, hipStream_t st);
+// kets per direct-mode work item of a class (the direct kernel's chunk)
+template <typename R>
+int direct_chunk(int la, int lb, int lc, int ld);
 
 // Launch one quartet class (bra class la,lb | ket class lc,ld).
 template <typename R>

@@ -353,42 +353,44 @@ void dev_antisym(int n, const double *M, double *E, hipStream_t st) {
 }
 
 // Shell-block density maxima for screening: Dsh[a][b] = max |D_mu nu| over
-// mu in a, nu in b, over up to two densities; then dmax = 2 max(Dsh).
-template <typename R>
-__global__ void shell_dmax_kernel(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const double *D0,
-                                  const double *D1, const double *D2, R *Dsh) {
-    const int a = blockIdx.y, b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nsh) return;
-    double m = 0.0;
-    for (int i = 0; i < sh_nf[a]; ++i)
-        for (int j = 0; j < sh_nf[b]; ++j) {
-            const size_t ij = (size_t)(sh_bf[a] + i) * nbf + sh_bf[b] + j;
-            m = fmax(m, fabs(D0[ij]));
-            if (D1) m = fmax(m, fabs(D1[ij]));
-            if (D2) m = fmax(m, fabs(D2[ij]));
-        }
-    Dsh[(size_t)a * nsh + b] = R(m);
+// mu in a, nu in b, over up to three densities; and dmax = 2 max(Dsh), each
+// wavefront's maximum merged with an integer atomic max on the bits (the
+// order of non-negative IEEE values is that of their bit patterns), dmax
+// cleared beforehand.
+__device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
+    atomicMax(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ void atomic_max_nonneg(float *p, float v) {
+    atomicMax(reinterpret_cast<unsigned int *>(p), __float_as_uint(v));
 }
 template <typename R>
-__global__ void __launch_bounds__(1024) max_kernel(int64_t n, const R *x, R scale, R *out) {
-    __shared__ R part[16];
-    R s = R(0);
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s = max(s, x[i]);
-    for (int o = 32; o >= 1; o >>= 1) s = max(s, __shfl_xor(s, o, 64));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        s = threadIdx.x < (int)(blockDim.x >> 6) ? part[threadIdx.x] : R(0);
-        for (int o = 32; o >= 1; o >>= 1) s = max(s, __shfl_xor(s, o, 64));
-        if (threadIdx.x == 0) *out = s * scale;
+__global__ void __launch_bounds__(64) shell_dmax_kernel(int nsh, int nbf, const int *sh_bf, const int *sh_nf,
+                                                        const double *D0, const double *D1, const double *D2, R *Dsh,
+                                                        R *dmax) {
+    const int a = blockIdx.y, b = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (b < nsh) {
+        for (int i = 0; i < sh_nf[a]; ++i)
+            for (int j = 0; j < sh_nf[b]; ++j) {
+                const size_t ij = (size_t)(sh_bf[a] + i) * nbf + sh_bf[b] + j;
+                m = fmax(m, fabs(D0[ij]));
+                if (D1) m = fmax(m, fabs(D1[ij]));
+                if (D2) m = fmax(m, fabs(D2[ij]));
+            }
+        Dsh[(size_t)a * nsh + b] = R(m);
     }
+    R w = R(m);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) w = max(w, __shfl_xor(w, o, 64));
+    if (threadIdx.x == 0) atomic_max_nonneg(dmax, R(2) * w);
 }
 template <typename R>
 void dev_shell_dmax(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const double *D0, const double *D1, R *Dsh,
                     R *dmax, hipStream_t st, const double *D2) {
     dim3 grid((nsh + 63) / 64, nsh);
-    shell_dmax_kernel<R><<<grid, 64, 0, st>>>(nsh, nbf, sh_bf, sh_nf, D0, D1, D2, Dsh);
-    max_kernel<R><<<1, 1024, 0, st>>>((int64_t)nsh * nsh, Dsh, R(2), dmax);
+    if constexpr (std::is_same<R, double>::value) dev_fill(1, 0.0, dmax, st);
+    else dev_fillf(1, 0.0f, dmax, st);
+    shell_dmax_kernel<R><<<grid, 64, 0, st>>>(nsh, nbf, sh_bf, sh_nf, D0, D1, D2, Dsh, dmax);
     PF_HIP(hipGetLastError());
 }
 template void dev_shell_dmax<double>(int, int, const int *, const int *, const double *, const double *, double *,
