@@ -85,7 +85,10 @@ template <int M, typename R>
 struct BoysLds {
     static constexpr int NT = BoysTraits<R>::terms;
     static constexpr int NC = M > 0 ? NT + 1 : NT;  // coefficients per grid point
-    static constexpr int STRIDE = NC;
+    // rows padded to 8 entries: the row offset is a shift (an integer
+    // multiply by a stride of 5-7 is a quarter-rate instruction on CDNA)
+    static constexpr int STRIDE = 8;
+    static_assert(NC <= STRIDE, "Boys row longer than its padded stride");
     static constexpr int SIZE = BOYS_NGRID * STRIDE;
     static constexpr int BASE = M > 0 ? M - 1 : 0;  // order of the stored polynomial
     static_assert(BASE + NC - 1 <= BOYS_MMAX, "Boys table too short for this order");
@@ -97,7 +100,7 @@ struct BoysLds {
             const int g = k / STRIDE, j = k - g * STRIDE;
             R f = R(1);
             for (int t = 2; t <= j; ++t) f *= R(t);
-            lds[k] = tab[g * (BOYS_MMAX + 1) + BASE + j] / f;
+            lds[k] = j < NC ? tab[g * (BOYS_MMAX + 1) + BASE + j] / f : R(0);
         }
     }
 
