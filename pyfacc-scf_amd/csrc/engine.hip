@@ -1126,6 +1126,7 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
     dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, Dt.p, D0 != Dt.p ? D0 : nullptr, Dsh64_.p, dmax64_.p,
                            st_, D1);
     std::array<DevBuf<double>, 3> gp;
+    DevBuf<int> queue(1);
     for (int c = 0; c < 3; ++c) {
         gp[c].alloc(std::max<size_t>(6 * (size_t)pairs_[c].n, 1));
         dev_fill(6 * (size_t)pairs_[c].n, 0.0, gp[c].p, st_);
@@ -1145,7 +1146,7 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
         g.Dsh = Dsh64_.p; g.dmax = dmax64_.p;
         g.gbra = gp[bc].p; g.gket = gp[kc].p;
         g.tab = d_tab_.p;
-        launch_grad2e_class(pairs_[bc].la, pairs_[bc].lb, pairs_[kc].la, pairs_[kc].lb, uhf ? 2 : 1, g, st_);
+        launch_grad2e_class(pairs_[bc].la, pairs_[bc].lb, pairs_[kc].la, pairs_[kc].lb, uhf ? 2 : 1, g, queue.p, st_);
     }
     std::vector<double> h1(9 * (size_t)na), hg(3 * (size_t)na, 0.0), h2(3 * (size_t)na, 0.0),
         hn(3 * (size_t)na, 0.0);

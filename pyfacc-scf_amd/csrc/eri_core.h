@@ -66,8 +66,9 @@ struct Eri {
 
     // Add one primitive quartet to acc[NE*NF] (the contracted [e0|f0]).
     // o2p = 1/(2p), o2q = 1/(2q).  LDS: tab is a workgroup table laid out by
-    // BoysLds<L, R>, otherwise the full table (orders 0 .. BOYS_MMAX).
-    template <bool LDS = false>
+    // BoysLds<MT, R> (MT >= L: a table of a higher order serves too),
+    // otherwise the full table (orders 0 .. BOYS_MMAX).
+    template <bool LDS = false, int MT = L>
     __device__ __forceinline__ static void prim(R p, R o2p, R Px, R Py, R Pz, R PAx, R PAy, R PAz, R Kb, R q, R o2q,
                                                 R Qx, R Qy, R Qz, R QCx, R QCy, R QCz, R Kk,
                                                 const R *__restrict__ tab, R *acc) {
@@ -77,8 +78,9 @@ struct Eri {
         const R rho = p * rop;
         const R PQ[3] = {Px - Qx, Py - Qy, Pz - Qz};
         const R T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-        R F[L + 1];
-        if constexpr (LDS) BoysLds<L, R>::eval(T, tab, F);
+        static_assert(MT >= L, "Boys table of too low an order");
+        R F[(LDS ? MT : L) + 1];
+        if constexpr (LDS) BoysLds<MT, R>::eval(T, tab, F);
         else boys_eval<L, R>(T, tab, F);
         const R pref = Kb * Kk * r;
         if constexpr (L == 0) {

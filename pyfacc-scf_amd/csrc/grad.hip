@@ -197,7 +197,7 @@ void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int
     PF_HIP(hipGetLastError());
 }
 
-#define PF_GRAD_EXTERN(a, b, c, d) extern template void launch_grad2e<a, b, c, d>(int, const GradArgs &, hipStream_t);
+#define PF_GRAD_EXTERN(a, b, c, d) extern template void launch_grad2e<a, b, c, d>(int, const GradArgs &, int *, hipStream_t);
 PF_GRAD_EXTERN(0, 0, 0, 0)
 PF_GRAD_EXTERN(1, 0, 0, 0)
 PF_GRAD_EXTERN(0, 0, 1, 0)
@@ -209,18 +209,18 @@ PF_GRAD_EXTERN(1, 0, 1, 1)
 PF_GRAD_EXTERN(1, 1, 1, 1)
 #undef PF_GRAD_EXTERN
 
-void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, hipStream_t st) {
+void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, int *queue, hipStream_t st) {
     const int key = ((la * 2 + lb) * 2 + lc) * 2 + ld;
     switch (key) {
-        case 0: launch_grad2e<0, 0, 0, 0>(nk, g, st); break;
-        case 8: launch_grad2e<1, 0, 0, 0>(nk, g, st); break;
-        case 2: launch_grad2e<0, 0, 1, 0>(nk, g, st); break;
-        case 10: launch_grad2e<1, 0, 1, 0>(nk, g, st); break;
-        case 12: launch_grad2e<1, 1, 0, 0>(nk, g, st); break;
-        case 3: launch_grad2e<0, 0, 1, 1>(nk, g, st); break;
-        case 14: launch_grad2e<1, 1, 1, 0>(nk, g, st); break;
-        case 11: launch_grad2e<1, 0, 1, 1>(nk, g, st); break;
-        case 15: launch_grad2e<1, 1, 1, 1>(nk, g, st); break;
+        case 0: launch_grad2e<0, 0, 0, 0>(nk, g, queue, st); break;
+        case 8: launch_grad2e<1, 0, 0, 0>(nk, g, queue, st); break;
+        case 2: launch_grad2e<0, 0, 1, 0>(nk, g, queue, st); break;
+        case 10: launch_grad2e<1, 0, 1, 0>(nk, g, queue, st); break;
+        case 12: launch_grad2e<1, 1, 0, 0>(nk, g, queue, st); break;
+        case 3: launch_grad2e<0, 0, 1, 1>(nk, g, queue, st); break;
+        case 14: launch_grad2e<1, 1, 1, 0>(nk, g, queue, st); break;
+        case 11: launch_grad2e<1, 0, 1, 1>(nk, g, queue, st); break;
+        case 15: launch_grad2e<1, 1, 1, 1>(nk, g, queue, st); break;
         default: throw Error("gradient: unsupported quartet class");
     }
     PF_HIP(hipGetLastError());

@@ -38,8 +38,9 @@ struct GradArgs {
 };
 
 // Contracted block of Eri<LA,LB,LC,LD> for bra pair i (prim data bp) and ket
-// pair j (prim data kp); AB / CD the pair separations.
-template <int LA, int LB, int LC, int LD>
+// pair j (prim data kp); AB / CD the pair separations; tab the workgroup's
+// Boys table of order MT (BoysLds<MT>).
+template <int LA, int LB, int LC, int LD, int MT>
 __device__ __forceinline__ void contracted_block(const double *__restrict__ bp, int bnpad, int bi, int kpb,
                                                  const double *__restrict__ kq, int knpad, int kj, int kpk,
                                                  double ABx, double ABy, double ABz, double CDx, double CDy,
@@ -55,9 +56,9 @@ __device__ __forceinline__ void contracted_block(const double *__restrict__ bp, 
         const double o2q = q[PF_O2P * knpad];
         for (int v = 0; v < kpb; ++v) {
             const double *b = bp + (size_t)v * PF_NFIELD * bnpad + bi;
-            E::prim(b[PF_P * bnpad], b[PF_O2P * bnpad], b[PF_PX * bnpad], b[PF_PY * bnpad], b[PF_PZ * bnpad],
-                    b[PF_AX * bnpad], b[PF_AY * bnpad], b[PF_AZ * bnpad], b[PF_K * bnpad], qq, o2q, Qx, Qy, Qz, QCx,
-                    QCy, QCz, Kk, tab, acc);
+            E::template prim<true, MT>(b[PF_P * bnpad], b[PF_O2P * bnpad], b[PF_PX * bnpad], b[PF_PY * bnpad],
+                                       b[PF_PZ * bnpad], b[PF_AX * bnpad], b[PF_AY * bnpad], b[PF_AZ * bnpad],
+                                       b[PF_K * bnpad], qq, o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, tab, acc);
         }
     }
     E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
@@ -70,21 +71,38 @@ __device__ __forceinline__ R gwave_sum(R v) {
     return v;
 }
 
+// Wavefronts per workgroup of the derivative kernel: they share one LDS copy
+// of the Boys table and take (bra pair, ket slice) items from a queue.
+constexpr int GRAD_W = 4;
+
+// Persistent: every wavefront takes items t = (bra pair i, ket slice s) from
+// the queue (*queue, cleared before the launch) until t >= B.n * nslice.
 template <int LA, int LB, int LC, int LD, int NK>
-__global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
+__global__ void __launch_bounds__(64 * GRAD_W) grad2e_kernel(GradArgs g, int nslice, int *queue) {
     constexpr int NA = nc(LA), NB = nc(LB), NC = nc(LC), ND = nc(LD);
     constexpr int NAp = nc(LA + 1), NBp = nc(LB + 1), NCp = nc(LC + 1);
-    const int i = blockIdx.x;
+    // one table of the derivative blocks' order L + 1 serves the L - 1 blocks too
+    constexpr int MT = LA + LB + LC + LD + 1;
+    using BT = BoysLds<MT, double>;
+    __shared__ double lds_boys[BT::SIZE];
+    BT::fill(lds_boys, g.tab, threadIdx.x, 64 * GRAD_W);
+    __syncthreads();
     const PairView<double> &B = g.bra;
     const PairView<double> &K = g.ket;
-    if (i >= B.n) return;
-    const int lane = threadIdx.x;
-    // blockIdx.y takes one slice of the ket list (slices of a multiple of 64
-    // covering all K.n kets), so a class with few bra pairs still fills the GPU;
+    const int lane = threadIdx.x & 63;
+    const int total = B.n * nslice;
+    // each item's slice of the ket list (slices of a multiple of 64 covering
+    // all K.n kets), so a class with few bra pairs still fills the GPU;
     // same-class lists end at ket i, so later slices have nothing to do
-    const int slice = ((K.n + (int)gridDim.y - 1) / (int)gridDim.y + 63) & ~63;
-    const int jbeg = (int)blockIdx.y * slice;
-    if (jbeg >= K.n || (g.same && jbeg > i)) return;
+    const int slice = ((K.n + nslice - 1) / nslice + 63) & ~63;
+    while (true) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= total) break;
+    const int i = t / nslice;
+    const int jbeg = (t - i * nslice) * slice;
+    if (jbeg >= K.n || (g.same && jbeg > i)) continue;
     const double Qi = B.Q[i];
     const double dm = g.dmax ? *g.dmax : 1.0;
     const double qthr = g.tau / (Qi * dm * dm);
@@ -96,7 +114,7 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
     }
     const int nket = g.same ? min(lo, i + 1) : lo;
     const int jend = min(nket, jbeg + slice);
-    if (jbeg >= jend) return;
+    if (jbeg >= jend) continue;
     const int a = B.sa[i], b = B.sb[i], oa = B.oa[i], ob = B.ob[i];
     const double ABx = B.AB[i], ABy = B.AB[B.npad + i], ABz = B.AB[2 * B.npad + i];
     const double deg_ab = (a == b) ? 0.5 : 1.0;
@@ -142,8 +160,8 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
         // ---- centre A: (a+1, b | c d) with 2 alpha weights, minus (a-1, b | c d)
         {
             double e1[NAp * NB * NC * ND];
-            contracted_block<LA + 1, LB, LC, LD>(g.bra_w, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, ABx, ABy, ABz,
-                                                  CDx, CDy, CDz, g.tab, e1);
+            contracted_block<LA + 1, LB, LC, LD, MT>(g.bra_w, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, ABx, ABy, ABz,
+                                                  CDx, CDy, CDz, lds_boys, e1);
 #pragma unroll
             for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -154,8 +172,8 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
                 }
             if constexpr (LA == 1) {
                 double e0[NB * NC * ND];
-                contracted_block<0, LB, LC, LD>(B.prim, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, ABx, ABy, ABz, CDx,
-                                                CDy, CDz, g.tab, e0);
+                contracted_block<0, LB, LC, LD, MT>(B.prim, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, ABx, ABy, ABz, CDx,
+                                                CDy, CDz, lds_boys, e0);
 #pragma unroll
                 for (int x = 0; x < 3; ++x)
 #pragma unroll
@@ -165,8 +183,8 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
         // ---- centre B: reversed pair (b+1, a | c d) with 2 beta weights, minus (b-1, a | c d)
         {
             double e1[NBp * NA * NC * ND];
-            contracted_block<LB + 1, LA, LC, LD>(g.bra_rw, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, -ABx, -ABy,
-                                                  -ABz, CDx, CDy, CDz, g.tab, e1);
+            contracted_block<LB + 1, LA, LC, LD, MT>(g.bra_rw, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, -ABx, -ABy,
+                                                  -ABz, CDx, CDy, CDz, lds_boys, e1);
 #pragma unroll
             for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
@@ -180,8 +198,8 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
                 }
             if constexpr (LB == 1) {
                 double e0[NA * NC * ND];
-                contracted_block<0, LA, LC, LD>(g.bra_r, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, -ABx, -ABy, -ABz,
-                                                CDx, CDy, CDz, g.tab, e0);
+                contracted_block<0, LA, LC, LD, MT>(g.bra_r, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, -ABx, -ABy, -ABz,
+                                                CDx, CDy, CDz, lds_boys, e0);
 #pragma unroll
                 for (int x = 0; x < 3; ++x)
 #pragma unroll
@@ -193,8 +211,8 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
         // ---- centre C: swapped quartet (c+1, d | a b) with 2 gamma weights, minus (c-1, d | a b)
         {
             double e1[NCp * ND * NA * NB];
-            contracted_block<LC + 1, LD, LA, LB>(g.ket_w, K.npad, j, K.kp, B.prim, B.npad, i, B.kp, CDx, CDy, CDz,
-                                                  ABx, ABy, ABz, g.tab, e1);
+            contracted_block<LC + 1, LD, LA, LB, MT>(g.ket_w, K.npad, j, K.kp, B.prim, B.npad, i, B.kp, CDx, CDy, CDz,
+                                                  ABx, ABy, ABz, lds_boys, e1);
 #pragma unroll
             for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
@@ -208,8 +226,8 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
                 }
             if constexpr (LC == 1) {
                 double e0[ND * NA * NB];
-                contracted_block<0, LD, LA, LB>(K.prim, K.npad, j, K.kp, B.prim, B.npad, i, B.kp, CDx, CDy, CDz, ABx,
-                                                ABy, ABz, g.tab, e0);
+                contracted_block<0, LD, LA, LB, MT>(K.prim, K.npad, j, K.kp, B.prim, B.npad, i, B.kp, CDx, CDy, CDz, ABx,
+                                                ABy, ABz, lds_boys, e0);
 #pragma unroll
                 for (int x = 0; x < 3; ++x)
 #pragma unroll
@@ -241,16 +259,32 @@ __global__ void __launch_bounds__(64) grad2e_kernel(GradArgs g) {
         for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
         if (lane == 0) atomicAdd(g.count, tot);
     }
+    }
 }
 
+// queue: a device int, cleared here (fill kernel) before the launch.
 template <int LA, int LB, int LC, int LD>
-void launch_grad2e(int nk, const GradArgs &g, hipStream_t st) {
+void launch_grad2e(int nk, const GradArgs &g, int *queue, hipStream_t st) {
     if (g.bra.n == 0) return;
-    // one workgroup per (bra pair, ket slice): slices of at least 64 kets, at
-    // most 64 of them
-    const dim3 grid(g.bra.n, std::max(1, std::min(64, (g.ket.n + 63) / 64)));
-    if (nk == 1) grad2e_kernel<LA, LB, LC, LD, 1><<<grid, 64, 0, st>>>(g);
-    else grad2e_kernel<LA, LB, LC, LD, 2><<<grid, 64, 0, st>>>(g);
+    // items (bra pair, ket slice): slices of at least 64 kets, at most 64 of them
+    const int nslice = std::max(1, std::min(64, (g.ket.n + 63) / 64));
+    const auto kern = nk == 1 ? grad2e_kernel<LA, LB, LC, LD, 1> : grad2e_kernel<LA, LB, LC, LD, 2>;
+    // persistent grid: all the workgroups that fit, worked out once per thread and device
+    struct Shape {
+        int dev = -1, grid = 0;
+    };
+    static thread_local Shape sh[2];
+    int dev = 0;
+    PF_HIP(hipGetDevice(&dev));
+    Shape &s = sh[nk == 1 ? 0 : 1];
+    if (s.dev != dev) {
+        int ncu = 0, per = 0;
+        PF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * GRAD_W, 0));
+        s = Shape{dev, std::max(1, per) * std::max(1, ncu)};
+    }
+    dev_fill_int(1, 0, queue, st);
+    kern<<<s.grid, 64 * GRAD_W, 0, st>>>(g, nslice, queue);
 }
 
 }  // namespace pf
