@@ -441,8 +441,12 @@ def main():
                              "(pf_options.profile = 1), run after the timed SCFs",
             "profiled_scf_s": {"mixed": prof_mx["t_total"], "fp64": prof_64["t_total"]},
             "flops_per_launch": fl,
+            # FP32 kernels: the build compiles with packed FP32 disabled, so the
+            # FP32 VALU rate it can reach is the unpacked one, equal to FP64's
+            "frac_of_unpacked_fp32_valu": achieved / PEAK_FP64_TFLOPS if prec == 1 else None,
             "fp64_full_build_dominant": {"kernel": flops.CLASS_NAMES[k64], "avg_launch_ms": avg64,
-                                         "tflops": fl64 / (avg64 * 1e-3) / 1e12},
+                                         "tflops": fl64 / (avg64 * 1e-3) / 1e12,
+                                         "frac": fl64 / (avg64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
         },
     }
     # the metric's accuracy bar: both SCFs converged and |E_mixed - E_FP64| <= 1e-6 Eh
