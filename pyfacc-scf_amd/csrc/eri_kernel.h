@@ -85,13 +85,20 @@ constexpr int jk_direct_occupancy() {
 // (ss|ss), where its registers cost a wavefront per SIMD (16 % slower).
 template <int LA, int LB, int LC, int LD, typename R>
 constexpr bool jk_prefetch() { return LA + LB + LC + LD != 0 || !std::is_same<R, double>::value; }
-// Kets per direct-mode work item (one workgroup): a few passes of every
-// wavefront, so that the per-item cost (queue atomic, row flush) is small
-// beside the item's integrals while items stay short enough to balance.
+// Kets per direct-mode work item (one workgroup): passes of every wavefront,
+// few enough that the last items of a launch do not leave the GPU waiting on
+// one workgroup (the FP32 builds are mostly increments, whose surviving
+// quartets crowd into the items of the largest bras) and enough that the
+// per-item cost (queue atomic, row flush) stays small beside the item's
+// integrals.  Measured on (H2O)64 (mixed / FP64 SCF, ms): 8-4-2 passes for
+// L <= 1, <= 3, 4 in both precisions 52.1 / 116.6; 4-2-1: 49.5 / 113.1;
+// FP64 4-2-1 with FP32 2-1-1: 49.1 / 113.1 (used); 2-1-1 in both
+// 50.2 / 122.1; one pass everywhere 52.7 / 134.7.
 template <int LA, int LB, int LC, int LD, typename R>
 constexpr int jk_chunk() {
     constexpr int L = LA + LB + LC + LD;
-    return 64 * jk_direct_waves<LA, LB, LC, LD, R>() * (L <= 1 ? 8 : (L <= 3 ? 4 : 2));
+    constexpr int passes = std::is_same<R, float>::value ? (L <= 1 ? 2 : 1) : (L <= 1 ? 4 : (L <= 3 ? 2 : 1));
+    return 64 * jk_direct_waves<LA, LB, LC, LD, R>() * passes;
 }
 
 // Contracted (ab|cd) block, eri[((ia*NB+ib)*NC+ic)*ND+id], of bra pair i
