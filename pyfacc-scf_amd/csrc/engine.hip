@@ -421,6 +421,13 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
     // direct kernels, the J'_cd sums in two after them
     const bool planned = mode == MODE_DIRECT;
     PlanArgs<R> plan;
+    // Work items grow with the basis: an item's fixed cost is the flush of its
+    // exchange rows, nbf long, while short items balance the end of a launch.
+    // Measured (mixed / FP64 SCF): (H2O)64, 448 bf, 49.2 / 112.8 ms at scale 1,
+    // 49.9 / 115.9 at 2; gly60, 1387 bf, 508 / 983 ms at 1, 474 / 944 at 3,
+    // 461 / 938 at 4; rna4, 1943 bf, 2937 / 7391 ms at 1, 2369 / 6537 at 4,
+    // 2388 / 6769 at 8.
+    const int chunk_scale = std::min(4, std::max(1, n / 448));
     if (planned) {
         plan.n = (int)classes_.size();
         plan.tau = a.tau;
@@ -434,7 +441,7 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
             if constexpr (std::is_same<R, double>::value) { c.bra = B.view64(); c.ket = K.view64(); }
             else { c.bra = B.view32(); c.ket = K.view32(); }
             c.same = classes_[q].bra == classes_[q].ket;
-            c.chunk = direct_chunk<R>(B.la, B.lb, K.la, K.lb);
+            c.chunk = direct_chunk<R>(B.la, B.lb, K.la, K.lb, chunk_scale);
             c.ncd = ncart(K.la) * ncart(K.lb);
             c.nket = plan_nket_[q].p; c.item_off = plan_off_[q].p; c.work = plan_work_[q].p;
             c.jhi = plan_jhi_[q].p; c.joff = plan_joff_[q].p;
@@ -460,6 +467,7 @@ void Engine::build_g(const double *D0, const double *D1, double *G0, double *G1,
         a.jcap = (jk_fallback & 2) ? 0 : jcap_[q];  // 0: no ket list fits, J'_cd through device atomics
         a.katomic = (jk_fallback & 1) ? 1 : 0;
         a.planned = planned ? 1 : 0;
+        a.chunk = planned ? plan.c[q].chunk : 0;
         a.ev0 = profile_classes ? ev[2 * q] : nullptr;
         a.ev1 = profile_classes ? ev[2 * q + 1] : nullptr;
         launch_jk_class<R>(B.la, B.lb, K.la, K.lb, uhf ? 2 : 1, mode, a, st_);
@@ -1126,7 +1134,9 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
     dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, Dt.p, D0 != Dt.p ? D0 : nullptr, Dsh64_.p, dmax64_.p,
                            st_, D1);
     std::array<DevBuf<double>, 3> gp;
-    DevBuf<int> queue(1);
+    int cap = 2;
+    for (int c = 0; c < 3; ++c) cap = std::max(cap, pairs_[c].n + 2);
+    DevBuf<int> work(cap);  // queue head and work plan of each class in turn
     for (int c = 0; c < 3; ++c) {
         gp[c].alloc(std::max<size_t>(6 * (size_t)pairs_[c].n, 1));
         dev_fill(6 * (size_t)pairs_[c].n, 0.0, gp[c].p, st_);
@@ -1136,7 +1146,7 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
         GradArgs g;
         g.bra = pairs_[bc].view64();
         g.ket = pairs_[kc].view64();
-        g.bra_w = gw_[bc].p; g.bra_rw = grw_[bc].p; g.bra_r = gr_[bc].p; g.ket_w = gw_[kc].p;
+        g.bra_w = gw_[bc].p; g.bra_rw = grw_[bc].p; g.bra_r = gr_[bc].p; g.ket_w = gw_[kc].p; g.ket_r = gr_[kc].p;
         g.same = bc == kc;
         g.tau = tau;
         g.nbf = n; g.nsh = nsh;
@@ -1146,7 +1156,7 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
         g.Dsh = Dsh64_.p; g.dmax = dmax64_.p;
         g.gbra = gp[bc].p; g.gket = gp[kc].p;
         g.tab = d_tab_.p;
-        launch_grad2e_class(pairs_[bc].la, pairs_[bc].lb, pairs_[kc].la, pairs_[kc].lb, uhf ? 2 : 1, g, queue.p, st_);
+        launch_grad2e_class(pairs_[bc].la, pairs_[bc].lb, pairs_[kc].la, pairs_[kc].lb, uhf ? 2 : 1, g, work.p, cap, st_);
     }
     std::vector<double> h1(9 * (size_t)na), hg(3 * (size_t)na, 0.0), h2(3 * (size_t)na, 0.0),
         hn(3 * (size_t)na, 0.0);

@@ -206,8 +206,9 @@ void launch_jcd_sums(PlanArgs<R> &p, hipStream_t st) {
 }
 
 template <typename R>
-int direct_chunk(int la, int lb, int lc, int ld) {
+int direct_chunk(int la, int lb, int lc, int ld, int scale) {
     const int key = ((la * 2 + lb) * 2 + lc) * 2 + ld;
+    return scale * [&] {
     switch (key) {
         case 0: return jk_chunk<0, 0, 0, 0, R>();
         case 8: return jk_chunk<1, 0, 0, 0, R>();
@@ -220,9 +221,10 @@ int direct_chunk(int la, int lb, int lc, int ld) {
         case 15: return jk_chunk<1, 1, 1, 1, R>();
         default: throw Error("jk: unsupported quartet class");
     }
+    }();
 }
-template int direct_chunk<double>(int, int, int, int);
-template int direct_chunk<float>(int, int, int, int);
+template int direct_chunk<double>(int, int, int, int, int);
+template int direct_chunk<float>(int, int, int, int, int);
 
 template void launch_plan<double>(PlanArgs<double> &, hipStream_t);
 template void launch_plan<float>(PlanArgs<float> &, hipStream_t);

@@ -649,7 +649,8 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
     if (!args.work || !args.item_off || !args.nket || !args.joff)
         throw Error("jk: direct mode needs its work-plan buffers");
     JKArgs<R> a = args;
-    a.chunk = jk_chunk<LA, LB, LC, LD, R>();
+    // a planned build counted its items with the plan's chunk
+    if (!a.planned || a.chunk <= 0) a.chunk = jk_chunk<LA, LB, LC, LD, R>();
     const auto kern = jk_direct_kernel<LA, LB, LC, LD, R, NK, DET>;
     // launch shape per (device, nbf), worked out once per thread: the
     // attribute and occupancy queries cost host time on every build otherwise

@@ -197,7 +197,62 @@ void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int
     PF_HIP(hipGetLastError());
 }
 
-#define PF_GRAD_EXTERN(a, b, c, d) extern template void launch_grad2e<a, b, c, d>(int, const GradArgs &, int *, hipStream_t);
+// Work plan of a derivative class, one workgroup of 1024 threads: bra pairs
+// in blocks of ib (pairs are sorted by Q descending, so a block's first pair
+// has the longest Schwarz ket prefix), and an item is (bra block, tile of 64
+// kets) for every tile of the first pair's prefix.  work[0] is the queue head
+// (cleared here), work[1 + b] the first item of block b, work[1 + nblk] the
+// number of items.
+__global__ void __launch_bounds__(1024) grad_plan_kernel(GradArgs g, int ib, int nblk, int *work) {
+    __shared__ int part[1024];
+    const PairView<double> &B = g.bra;
+    const PairView<double> &K = g.ket;
+    const int tid = threadIdx.x;
+    const int per = (nblk + 1023) / 1024;
+    const int b0 = min(nblk, tid * per), b1 = min(nblk, b0 + per);
+    const double dm = g.dmax ? *g.dmax : 1.0;
+    int *off = work + 1;
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) {
+        const int i0 = b * ib;
+        const double qthr = g.tau / (B.Q[i0] * dm * dm);
+        int lo = 0, hi = K.n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (K.Q[mid] >= qthr) lo = mid + 1;
+            else hi = mid;
+        }
+        // same-class lists stop at the block's last bra pair
+        const int nket = g.same ? min(lo, min(B.n, i0 + ib)) : lo;
+        off[b] = (nket + 63) / 64;
+        sum += off[b];
+    }
+    part[tid] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;  // exclusive prefix of this thread's blocks
+    for (int b = b0; b < b1; ++b) {
+        const int c = off[b];
+        off[b] = run;
+        run += c;
+    }
+    if (tid == 1023) {
+        off[nblk] = part[1023];
+        work[0] = 0;
+    }
+}
+
+void launch_grad_plan(const GradArgs &g, int ib, int nblk, int *work, hipStream_t st) {
+    grad_plan_kernel<<<1, 1024, 0, st>>>(g, ib, nblk, work);
+    PF_HIP(hipGetLastError());
+}
+
+#define PF_GRAD_EXTERN(a, b, c, d) extern template void launch_grad2e<a, b, c, d>(int, const GradArgs &, int *, int, hipStream_t);
 PF_GRAD_EXTERN(0, 0, 0, 0)
 PF_GRAD_EXTERN(1, 0, 0, 0)
 PF_GRAD_EXTERN(0, 0, 1, 0)
@@ -209,18 +264,18 @@ PF_GRAD_EXTERN(1, 0, 1, 1)
 PF_GRAD_EXTERN(1, 1, 1, 1)
 #undef PF_GRAD_EXTERN
 
-void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, int *queue, hipStream_t st) {
+void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, int *work, int cap, hipStream_t st) {
     const int key = ((la * 2 + lb) * 2 + lc) * 2 + ld;
     switch (key) {
-        case 0: launch_grad2e<0, 0, 0, 0>(nk, g, queue, st); break;
-        case 8: launch_grad2e<1, 0, 0, 0>(nk, g, queue, st); break;
-        case 2: launch_grad2e<0, 0, 1, 0>(nk, g, queue, st); break;
-        case 10: launch_grad2e<1, 0, 1, 0>(nk, g, queue, st); break;
-        case 12: launch_grad2e<1, 1, 0, 0>(nk, g, queue, st); break;
-        case 3: launch_grad2e<0, 0, 1, 1>(nk, g, queue, st); break;
-        case 14: launch_grad2e<1, 1, 1, 0>(nk, g, queue, st); break;
-        case 11: launch_grad2e<1, 0, 1, 1>(nk, g, queue, st); break;
-        case 15: launch_grad2e<1, 1, 1, 1>(nk, g, queue, st); break;
+        case 0: launch_grad2e<0, 0, 0, 0>(nk, g, work, cap, st); break;
+        case 8: launch_grad2e<1, 0, 0, 0>(nk, g, work, cap, st); break;
+        case 2: launch_grad2e<0, 0, 1, 0>(nk, g, work, cap, st); break;
+        case 10: launch_grad2e<1, 0, 1, 0>(nk, g, work, cap, st); break;
+        case 12: launch_grad2e<1, 1, 0, 0>(nk, g, work, cap, st); break;
+        case 3: launch_grad2e<0, 0, 1, 1>(nk, g, work, cap, st); break;
+        case 14: launch_grad2e<1, 1, 1, 0>(nk, g, work, cap, st); break;
+        case 11: launch_grad2e<1, 0, 1, 1>(nk, g, work, cap, st); break;
+        case 15: launch_grad2e<1, 1, 1, 1>(nk, g, work, cap, st); break;
         default: throw Error("gradient: unsupported quartet class");
     }
     PF_HIP(hipGetLastError());

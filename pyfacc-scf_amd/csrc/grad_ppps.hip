@@ -2,5 +2,5 @@
 #include "grad_kernel.h"
 
 namespace pf {
-template void launch_grad2e<1, 1, 1, 0>(int, const GradArgs &, int *, hipStream_t);
+template void launch_grad2e<1, 1, 1, 0>(int, const GradArgs &, int *, int, hipStream_t);
 }  // namespace pf

@@ -130,7 +130,7 @@ template <typename R>
 void launch_jcd_sums(PlanArgs<R> &p, hipStream_t st);
 // kets per direct-mode work item of a class (the direct kernel's chunk)
 template <typename R>
-int direct_chunk(int la, int lb, int lc, int ld);
+int direct_chunk(int la, int lb, int lc, int ld, int scale);
 
 // Launch one quartet class (bra class la,lb | ket class lc,ld).
 template <typename R>
@@ -140,16 +140,13 @@ void launch_jk_class(int la, int lb, int lc, int ld, int nk, int mode, const JKA
 template <typename R>
 void launch_schwarz(int la, int lb, const PairView<R> &pv, const R *tab, R *Q, hipStream_t st);
 
-// n ints set to v (a fill kernel: safe inside captured graphs)
-void dev_fill_int(int64_t n, int v, int *x, hipStream_t st);
-
 // ---------------------------------------------------------------- gradient
 struct GradArgs;
 void launch_grad1e(int la, int lb, const Shell *shells, const int *pa, const int *pb, int npairs, int nbf, int natoms,
                    const double *axyz, const int *Z, const double *tab, const double *D, const double *W,
                    double *gT, double *gV, double *gS, hipStream_t st);
 // queue: a device int the persistent derivative kernel takes its items from
-void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, int *queue, hipStream_t st);
+void launch_grad2e_class(int la, int lb, int lc, int ld, int nk, const GradArgs &g, int *work, int cap, hipStream_t st);
 
 // ---------------------------------------------------------------- dense FP64
 // C = alpha op(A) op(B) + beta C, row-major n x n (lda = ldb = ldc = n), MFMA f64.

@@ -248,16 +248,6 @@ __global__ void fillf_kernel(int64_t n, float v, float *x) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) x[i] = v;
 }
-__global__ void fill_int_kernel(int64_t n, int v, int *x) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
-}
-void dev_fill_int(int64_t n, int v, int *x, hipStream_t st) {
-    if (n <= 0) return;
-    fill_int_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, v, x);
-    PF_HIP(hipGetLastError());
-}
-
 void dev_fill(int64_t n, double v, double *x, hipStream_t st) {
     if (n <= 0) return;
     fill_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, v, x);
