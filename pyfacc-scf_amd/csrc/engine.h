@@ -38,6 +38,11 @@ void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shel
                   const std::vector<std::pair<int, int>> &list, double kref, double &bound_max, bool want32,
                   hipStream_t st);
 
+// The gradient's three primitive-pair sets of ps (kp * PF_NFIELD * npad
+// doubles each, primitives in shell order): P - A with K 2 alpha, P - B with
+// K 2 beta, P - B with K.
+void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *rw, double *r, hipStream_t st);
+
 struct QuartetClass {
     int bra, ket;  // indices into Engine::pairs_
     int slot;      // unordered class: (ss|ss) (ps|ss) (ps|ps) (pp|ss) (pp|ps) (pp|pp)
@@ -186,6 +191,9 @@ class Engine {
     // gradient pair data: 2 alpha weighted, reversed 2 beta weighted, reversed
     bool grad_ready_ = false;
     std::array<DevBuf<double>, 3> gw_, grw_, gr_;
+    // one-electron derivative pairs: ordered shell pairs by (la, lb) = (0,0), (0,1), (1,0), (1,1)
+    DevBuf<int> g1pairs_;                // the four classes' (a, b) lists, a's then b's per class
+    std::array<int, 4> g1n_{}, g1off_{};
 };
 
 // Thread-local error text for the C-ABI.

@@ -1047,46 +1047,29 @@ void Engine::setup_grad_pairs() {
     if (grad_ready_) return;
     setup_pairs();
     const auto &sh = basis_.shells;
-    const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
     for (int c = 0; c < 3; ++c) {
         const PairSet &ps = pairs_[c];
         const size_t sz = (size_t)ps.kp * PF_NFIELD * ps.npad;
-        std::vector<double> w(sz, 0.0), rw(sz, 0.0), r(sz, 0.0);
-        for (int u = 0; u < ps.kp; ++u)
-            for (int i = 0; i < ps.npad; ++i) {
-                const size_t o = ((size_t)u * PF_NFIELD + PF_P) * ps.npad + i;
-                w[o] = rw[o] = r[o] = 1.0;
-                const size_t o2 = ((size_t)u * PF_NFIELD + PF_O2P) * ps.npad + i;
-                w[o2] = rw[o2] = r[o2] = 0.5;
-            }
-        for (int i = 0; i < ps.n; ++i) {
-            const Shell &A = sh[ps.sa[i]], &B = sh[ps.sb[i]];
-            double r2 = 0.0;
-            for (int d = 0; d < 3; ++d) r2 += (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]);
-            int u = 0;
-            for (int x = 0; x < A.np; ++x)
-                for (int y = 0; y < B.np; ++y, ++u) {
-                    const double a = A.exp[x], b = B.exp[y], p = a + b;
-                    const double K = kfac * A.coef[x] * B.coef[y] * std::exp(-a * b / p * r2) / p;
-                    for (auto *arr : {&w, &rw, &r}) {
-                        double *col = arr->data() + (size_t)u * PF_NFIELD * ps.npad + i;
-                        col[PF_P * ps.npad] = p;
-                        for (int d = 0; d < 3; ++d) {
-                            const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
-                            col[(PF_PX + d) * ps.npad] = P;
-                            col[(PF_AX + d) * ps.npad] = P - (arr == &w ? A.xyz[d] : B.xyz[d]);
-                        }
-                        col[PF_K * ps.npad] = arr == &w ? 2.0 * a * K : (arr == &rw ? 2.0 * b * K : K);
-                        col[PF_O2P * ps.npad] = 0.5 / p;
-                    }
-                }
-            for (; u < ps.kp; ++u)
-                for (auto *arr : {&w, &rw, &r}) (*arr)[((size_t)u * PF_NFIELD + PF_K) * ps.npad + i] = 0.0;
-        }
-        gw_[c].upload(w.data(), sz, st_);
-        grw_[c].upload(rw.data(), sz, st_);
-        gr_[c].upload(r.data(), sz, st_);
+        gw_[c].alloc(std::max<size_t>(sz, 1));
+        grw_[c].alloc(std::max<size_t>(sz, 1));
+        gr_[c].alloc(std::max<size_t>(sz, 1));
+        fill_grad_pairset(ps, d_shells_.p, gw_[c].p, grw_[c].p, gr_[c].p, st_);
     }
+    const int nsh = nshell();
+    std::vector<int> flat;
+    for (int la = 0; la <= 1; ++la)
+        for (int lb = 0; lb <= 1; ++lb) {
+            std::vector<int> pa, pb;
+            for (int a = 0; a < nsh; ++a)
+                for (int b = 0; b < nsh; ++b)
+                    if (sh[a].l == la && sh[b].l == lb) { pa.push_back(a); pb.push_back(b); }
+            const int k = 2 * la + lb;
+            g1n_[k] = (int)pa.size();
+            g1off_[k] = (int)flat.size();
+            flat.insert(flat.end(), pa.begin(), pa.end());
+            flat.insert(flat.end(), pb.begin(), pb.end());
+        }
+    g1pairs_.upload(flat.data(), flat.size(), st_);
     grad_ready_ = true;
 }
 
@@ -1113,23 +1096,14 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
     else PF_HIP(hipMemcpyAsync(Dt.p, D0, sizeof(double) * n2, hipMemcpyDeviceToDevice, st_));
     dev_fill(9 * (size_t)na, 0.0, grad.p, st_);  // kinetic, nuclear attraction, overlap
     // ---- one-electron part over ordered shell pairs, by (la, lb)
-    {
-        const auto &sh = basis_.shells;
-        for (int la = 0; la <= 1; ++la)
-            for (int lb = 0; lb <= 1; ++lb) {
-                std::vector<int> pa, pb;
-                for (int a = 0; a < nsh; ++a)
-                    for (int b = 0; b < nsh; ++b)
-                        if (sh[a].l == la && sh[b].l == lb) { pa.push_back(a); pb.push_back(b); }
-                if (pa.empty()) continue;
-                DevBuf<int> dpa, dpb;
-                dpa.upload(pa.data(), pa.size(), st_);
-                dpb.upload(pb.data(), pb.size(), st_);
-                launch_grad1e(la, lb, d_shells_.p, dpa.p, dpb.p, (int)pa.size(), n, na, d_axyz_.p, d_Z_.p,
-                              d_tab_.p, Dt.p, W.p, grad.p, grad.p + 3 * na, grad.p + 6 * na, st_);
-                PF_HIP(hipStreamSynchronize(st_));
-            }
-    }
+    for (int la = 0; la <= 1; ++la)
+        for (int lb = 0; lb <= 1; ++lb) {
+            const int k = 2 * la + lb;
+            if (g1n_[k] == 0) continue;
+            const int *pa = g1pairs_.p + g1off_[k];
+            launch_grad1e(la, lb, d_shells_.p, pa, pa + g1n_[k], g1n_[k], n, na, d_axyz_.p, d_Z_.p, d_tab_.p, Dt.p, W.p,
+                          grad.p, grad.p + 3 * na, grad.p + 6 * na, st_);
+        }
     // ---- two-electron part
     dev_shell_dmax<double>(nsh, n, d_sh_bf_.p, d_sh_nf_.p, Dt.p, D0 != Dt.p ? D0 : nullptr, Dsh64_.p, dmax64_.p,
                            st_, D1);

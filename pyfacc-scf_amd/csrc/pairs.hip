@@ -109,7 +109,58 @@ __global__ void __launch_bounds__(1024) max_reduce_kernel(int n, const double *x
 
 int round64(int n) { return ((n + 63) / 64) * 64; }
 
+// Gradient primitive-pair sets, primitives in shell order (x over A, y over
+// B), one thread per pair slot: w (P - A, K 2 alpha), rw (P - B, K 2 beta),
+// r (P - B, K).  Padding slots and primitives: p = 1, 1/2p = 1/2, K = 0.
+__global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict__ sh, const int *__restrict__ sa,
+                                                         const int *__restrict__ sb, int n, int npad, int kp,
+                                                         double *w, double *rw, double *r) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npad) return;
+    const double kfac = sqrt(2.0) * pow(M_PI, 1.25);
+    int u = 0;
+    if (i < n) {
+        const Shell &A = sh[sa[i]], &B = sh[sb[i]];
+        double r2 = 0.0;
+        for (int d = 0; d < 3; ++d) r2 += (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]);
+        for (int x = 0; x < A.np; ++x)
+            for (int y = 0; y < B.np; ++y, ++u) {
+                const double a = A.exp[x], b = B.exp[y], p = a + b;
+                const double K = kfac * A.coef[x] * B.coef[y] * exp(-a * b / p * r2) / p;
+                const size_t o = (size_t)u * PF_NFIELD * npad + i;
+                double *cw = w + o, *crw = rw + o, *cr = r + o;
+                cw[PF_P * npad] = crw[PF_P * npad] = cr[PF_P * npad] = p;
+                cw[PF_O2P * npad] = crw[PF_O2P * npad] = cr[PF_O2P * npad] = 0.5 / p;
+                for (int d = 0; d < 3; ++d) {
+                    const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
+                    cw[(PF_PX + d) * npad] = crw[(PF_PX + d) * npad] = cr[(PF_PX + d) * npad] = P;
+                    cw[(PF_AX + d) * npad] = P - A.xyz[d];
+                    crw[(PF_AX + d) * npad] = cr[(PF_AX + d) * npad] = P - B.xyz[d];
+                }
+                cw[PF_K * npad] = 2.0 * a * K;
+                crw[PF_K * npad] = 2.0 * b * K;
+                cr[PF_K * npad] = K;
+                cw[PF_BND * npad] = crw[PF_BND * npad] = cr[PF_BND * npad] = 0.0;
+            }
+    }
+    for (; u < kp; ++u) {
+        const size_t o = (size_t)u * PF_NFIELD * npad + i;
+        for (double *c : {w + o, rw + o, r + o}) {
+            for (int f = 0; f < PF_NFIELD; ++f) c[f * npad] = 0.0;
+            c[PF_P * npad] = 1.0;
+            c[PF_O2P * npad] = 0.5;
+        }
+    }
+}
+
 }  // namespace
+
+void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *rw, double *r, hipStream_t st) {
+    if (ps.npad == 0) return;
+    grad_pairs_kernel<<<(ps.npad + 255) / 256, 256, 0, st>>>(d_shells, ps.d_sa.p, ps.d_sb.p, ps.n, ps.npad, ps.kp, w,
+                                                             rw, r);
+    PF_HIP(hipGetLastError());
+}
 
 void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shells,
                   const std::vector<std::pair<int, int>> &list, double kref, double &bound_max, bool want32,
