@@ -376,6 +376,24 @@ def main():
         "tflops": float(flops.class_flops(c, q_64[0, c] / nb64[c], orient=orient64[c],
                                           prim_quartets=p_64[0, c] / nb64[c]) /
                         max(ms_64[0, c] / nb64[c] * 1e-3, 1e-12) / 1e12)} for c in range(6)}
+    for v in per_class.values():
+        v["frac"] = v["tflops"] / PEAK_FP64_TFLOPS
+    # every J/K class of the timed mixed SCF (FP32 increments and the FP64
+    # anchor and switch builds), from the profiled mixed SCF
+    mixed_per_class = {}
+    for pr in (0, 1):
+        for c in range(6):
+            nl = int(n_mx[pr, c])
+            if nl == 0:
+                continue
+            ms = float(ms_mx[pr, c] / nl)
+            f = flops.class_flops(c, q_mx[pr, c] / nl, ndens=1, taylor_terms=7 if pr == 0 else 5, orient=orient[c],
+                                  prim_quartets=p_mx[pr, c] / nl)
+            tf = f / (ms * 1e-3) / 1e12
+            mixed_per_class[f"{flops.CLASS_NAMES[c]} {'fp64' if pr == 0 else 'fp32'}"] = {
+                "launches": nl, "avg_launch_ms": ms, "tflops": tf,
+                "frac": tf / (PEAK_FP64_TFLOPS if pr == 0 else 157.3),
+                "frac_of_unpacked_fp32_valu": tf / PEAK_FP64_TFLOPS if pr == 1 else None}
 
     out = {
         "metric": "SCF time-to-converge & Fock builds/s, mixed vs FP64 GPU at |dE|<=1e-6 Eh",
@@ -425,6 +443,7 @@ def main():
                                               (sum(v["ms_per_build"] for v in per_class.values()) * 1e-3)),
         },
         "fp64_build_per_class": per_class,
+        "mixed_scf_per_class": mixed_per_class,
         "roofline": {
             # the J/K kernels are vector-ALU (VALU) FP64/FP32 arithmetic: no MFMA
             "bound": "valu",
