@@ -85,6 +85,17 @@ constexpr int jk_direct_occupancy() {
 // (ss|ss), where its registers cost a wavefront per SIMD (16 % slower).
 template <int LA, int LB, int LC, int LD, typename R>
 constexpr bool jk_prefetch() { return LA + LB + LC + LD != 0 || !std::is_same<R, double>::value; }
+// FP32 builds (the mixed path's increments): a ket primitive whose bound
+// times the bra's largest bound times the quartet's density factor dq is
+// under JK_PCUT_REL tau ends the lane's primitive loop (primitives are sorted
+// by bound), so each primitive quartet left out is below JK_PCUT_REL tau / dq
+// in the integral and below JK_PCUT_REL tau in J and K (up to the p-function
+// polynomial factors), against tau for a whole quartet left out by the
+// screen.  FP64 builds keep the fixed cut PRIM_EPS.  Measured on (H2O)64
+// (mixed SCF): 49.1 -> 47.4 ms with 1e-2, the energy moving 3e-10 Eh; 3e-2
+// gives the same time and moves it 2.6e-9 Eh.
+constexpr double JK_PCUT_REL = 1e-2;
+
 // Kets per direct-mode work item (one workgroup): passes of every wavefront,
 // few enough that the last items of a launch do not leave the GPU waiting on
 // one workgroup (the FP32 builds are mostly increments, whose surviving
@@ -105,7 +116,8 @@ constexpr int jk_chunk() {
 // (uniform; its primitive data through bprim) and ket pair j (this lane).
 template <int LA, int LB, int LC, int LD, typename R, bool PREFETCH = true>
 __device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restrict__ bprim, int i, R ABx, R ABy,
-                                          R ABz, const PairView<R> &K, int j, const R *lds_boys, R *eri) {
+                                          R ABz, const PairView<R> &K, int j, const R *lds_boys, R *eri,
+                                          R pcut = R(0)) {
     using E = Eri<LA, LB, LC, LD, R>;
     R acc[E::NACC];
 #pragma unroll
@@ -114,7 +126,10 @@ __device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restri
     // and of those only the ones whose bound times the bra's largest reaches
     // the cut: ket primitives are sorted by bound, so the loop stops there)
     const int kpk = K.kpn[j], kpb = __builtin_amdgcn_readfirstlane(B.kpn[i]);
-    const R bmax = bprim[(size_t)PF_BND * B.npad + i], cut = B.peps;  // bra primitive 0 has the largest bound
+    // bra primitive 0 has the largest bound; pcut: this quartet's own cut (0: none)
+    const R bmax = bprim[(size_t)PF_BND * B.npad + i];
+    R cut = B.peps;
+    if constexpr (std::is_same<R, float>::value) cut = max(cut, pcut);
     // this lane's ket primitive fields, the next primitive's loaded while the
     // current one runs through the bra primitives (hides the load latency)
     struct KetPrim {
@@ -385,6 +400,9 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
     constexpr int NA = E::NA, NB = E::NB, NINT = E::NINT, NAB = NA + NB;
     __shared__ R lds_boys[BT::SIZE];
     __shared__ int lds_list[W][128];
+    // FP32 builds: each queued ket's density factor, for its primitive cut
+    constexpr bool PCUT = std::is_same<R, float>::value;
+    __shared__ float lds_dq[PCUT ? W : 1][PCUT ? 128 : 1];
     __shared__ int s_item;
     extern __shared__ double lds_acc[];  // [NK][NAB][nbf] exchange rows if g.krows, then [NA*NB] J'_ab
     BT::fill(lds_boys, g.tab, threadIdx.x, BLOCK);
@@ -396,6 +414,7 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
     double *ljab = lds_acc + nrow;
     for (int e = threadIdx.x; e < nrow + NA * NB; e += BLOCK) lds_acc[e] = 0.0;
     int *list = lds_list[wave];
+    float *dql = lds_dq[PCUT ? wave : 0];
     const bool compact = g.Dsh != nullptr;
     const int total = g.work[1];
     constexpr int NCD = E::NC * E::ND;
@@ -435,17 +454,20 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
         int next = wb, queued = 0;  // wavefront-uniform
         while (true) {
             int j;
+            R jdq = R(0);  // FP32: the queued ket's density factor
             if (compact) {
                 const int ns = g.nsh;
                 while (queued < 64 && next < we) {
                     const int jj = next + lane;
                     bool keep = false;
+                    R dqs = R(0);
                     if (jj < we) {
                         const int c = K.sa[jj], d = K.sb[jj];
                         R dq = R(2) * max(g.Dsh[a * ns + b], g.Dsh[c * ns + d]);
                         dq = max(dq, max(max(g.Dsh[b * ns + d], g.Dsh[b * ns + c]),
                                          max(g.Dsh[a * ns + d], g.Dsh[a * ns + c])));
                         keep = Qi * K.Q[jj] * dq >= g.tau;
+                        if constexpr (PCUT) dqs = dq;
                         if (!keep && jstore) {  // a screened ket adds nothing to its J'_cd column
                             double *jb = g.jbuf + (g.joff[i] + jj) * NCD;
 #pragma unroll
@@ -457,6 +479,7 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
                         const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                         list[queued + pos] = jj;
+                        if constexpr (PCUT) dql[queued + pos] = dqs;
                     }
                     queued += __popcll(m);
                     next += 64;
@@ -467,8 +490,11 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
                 if (queued == 0) break;
                 const int take = min(queued, 64);
                 j = lane < take ? list[lane] : -1;
+                if constexpr (PCUT) jdq = lane < take ? dql[lane] : R(0);
                 // entries 64 .. queued-1 move down; lane l alone reads and writes slot l
                 if (lane + 64 < queued) list[lane] = list[lane + 64];
+                if constexpr (PCUT)
+                    if (lane + 64 < queued) dql[lane] = dql[lane + 64];
                 queued -= take;
             } else {
                 if (next >= we) break;
@@ -479,8 +505,11 @@ __global__ void __launch_bounds__((64 * jk_direct_waves<LA, LB, LC, LD, R>()), (
             const int c = K.sa[j], d = K.sb[j], oc = K.oa[j], od = K.ob[j];
             ++counted;
             R eri[NINT];
+            // FP32: primitive quartets whose bound times the quartet's density
+            // factor is under JK_PCUT_REL tau are left out
+            const R pcut = (PCUT && jdq > R(0)) ? g.tau * R(JK_PCUT_REL) / jdq : R(0);
             pcounted += eri_block<LA, LB, LC, LD, R, jk_prefetch<LA, LB, LC, LD, R>()>(B, bprim, i, ABx, ABy, ABz, K, j,
-                                                                                   lds_boys, eri);
+                                                                                   lds_boys, eri, pcut);
             const R deg = deg_ab * ((c == d) ? R(0.5) : R(1)) * ((g.same && i == j) ? R(0.5) : R(1));
             double jcd[NCD];  // this quartet's J'_cd block
             const auto jadd = [&](int row, int col, R v) { jcd[(row - oc) * E::ND + (col - od)] = double(v); };
