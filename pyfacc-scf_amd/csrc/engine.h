@@ -23,8 +23,8 @@ struct PairSet {
     std::vector<int> sa, sb;   // host copies, sorted by Q descending
     std::vector<double> Q;
     DevBuf<int> d_sa, d_sb, d_oa, d_ob, d_kpn;
-    DevBuf<double> d_Q, d_AB, d_prim;
-    DevBuf<float> f_Q, f_AB, f_prim;
+    DevBuf<double> d_Q, d_AB, d_prim, d_aos;
+    DevBuf<float> f_Q, f_AB, f_prim, f_aos;
     PairView<double> view64() const;
     PairView<float> view32() const;
 };

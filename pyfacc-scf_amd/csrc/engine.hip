@@ -44,7 +44,7 @@ PairView<double> PairSet::view64() const {
     PairView<double> v;
     v.n = n; v.npad = npad; v.kp = kp;
     v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
-    v.Q = d_Q.p; v.AB = d_AB.p; v.prim = d_prim.p;
+    v.Q = d_Q.p; v.AB = d_AB.p; v.prim = d_prim.p; v.aos = d_aos.p;
     v.kpn = d_kpn.p;
     v.peps = peps;
     return v;
@@ -53,7 +53,7 @@ PairView<float> PairSet::view32() const {
     PairView<float> v;
     v.n = n; v.npad = npad; v.kp = kp;
     v.sa = d_sa.p; v.sb = d_sb.p; v.oa = d_oa.p; v.ob = d_ob.p;
-    v.Q = f_Q.p; v.AB = f_AB.p; v.prim = f_prim.p;
+    v.Q = f_Q.p; v.AB = f_AB.p; v.prim = f_prim.p; v.aos = f_aos.p;
     v.kpn = d_kpn.p;
     v.peps = float(peps);
     return v;

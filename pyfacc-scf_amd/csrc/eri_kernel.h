@@ -127,7 +127,8 @@ __device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restri
     // the cut: ket primitives are sorted by bound, so the loop stops there)
     const int kpk = K.kpn[j], kpb = __builtin_amdgcn_readfirstlane(B.kpn[i]);
     // bra primitive 0 has the largest bound; pcut: this quartet's own cut (0: none)
-    const R bmax = bprim[(size_t)PF_BND * B.npad + i];
+    const R *bpair = bprim + (size_t)i * B.kp * PF_AOS;  // pair-major copy: pair i's primitives contiguous
+    const R bmax = bpair[PF_BND];
     R cut = B.peps;
     if constexpr (std::is_same<R, float>::value) cut = max(cut, pcut);
     // this lane's ket primitive fields, the next primitive's loaded while the
@@ -149,10 +150,9 @@ __device__ __forceinline__ int eri_block(const PairView<R> &B, const R *__restri
         if (PREFETCH && u + 1 < kpk) nx = load(u + 1);
         for (int v = 0; v < kpb; ++v) {
             // bra data: read-only in these kernels and wavefront-uniform -> scalar loads
-            const R *bp = bprim + (size_t)v * PF_NFIELD * B.npad + i;
-            E::template prim<true>(bp[PF_P * B.npad], bp[PF_O2P * B.npad], bp[PF_PX * B.npad], bp[PF_PY * B.npad],
-                                   bp[PF_PZ * B.npad], bp[PF_AX * B.npad], bp[PF_AY * B.npad], bp[PF_AZ * B.npad],
-                                   bp[PF_K * B.npad], c.q, c.o2q, c.Qx, c.Qy, c.Qz, c.QCx, c.QCy, c.QCz, c.Kk,
+            const R *bp = bpair + (size_t)v * PF_AOS;
+            E::template prim<true>(bp[PF_P], bp[PF_O2P], bp[PF_PX], bp[PF_PY], bp[PF_PZ], bp[PF_AX], bp[PF_AY],
+                                   bp[PF_AZ], bp[PF_K], c.q, c.o2q, c.Qx, c.Qy, c.Qz, c.QCx, c.QCy, c.QCz, c.Kk,
                                    lds_boys, acc);
         }
     }
@@ -717,7 +717,7 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
         jk_scan_kernel<<<1, 1024, 0, st>>>(a.bra.n, a.item_off, a.nket, a.joff, a.jbuf ? a.jcap : -1, a.work);
     }
     if (a.ev0) PF_HIP(hipEventRecord(a.ev0, st));
-    kern<<<grid, 64 * W, dyn, st>>>(a, a.bra.prim, a.item_off);
+    kern<<<grid, 64 * W, dyn, st>>>(a, a.bra.aos, a.item_off);
     if (a.ev1) PF_HIP(hipEventRecord(a.ev1, st));
     if (a.jbuf && !a.planned) {
         constexpr int NCD = nc(LC) * nc(LD);
@@ -731,11 +731,11 @@ void launch_direct(const JKArgs<R> &args, hipStream_t st) {
 template <int LA, int LB, int LC, int LD, typename R>
 void launch_nk_stored(int nk, int mode, const JKArgs<R> &args, dim3 grid, dim3 block, hipStream_t st) {
     if constexpr (std::is_same<R, double>::value) {
-        if (mode == MODE_STORE) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_STORE><<<grid, block, 0, st>>>(args, args.bra.prim); return; }
-        if (mode == MODE_PACKED) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_PACKED><<<grid, block, 0, st>>>(args, args.bra.prim); return; }
+        if (mode == MODE_STORE) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_STORE><<<grid, block, 0, st>>>(args, args.bra.aos); return; }
+        if (mode == MODE_PACKED) { jk_kernel<LA, LB, LC, LD, R, 1, MODE_PACKED><<<grid, block, 0, st>>>(args, args.bra.aos); return; }
         if (mode == MODE_LOAD) {
-            if (nk == 1) jk_kernel<LA, LB, LC, LD, R, 1, MODE_LOAD><<<grid, block, 0, st>>>(args, args.bra.prim);
-            else jk_kernel<LA, LB, LC, LD, R, 2, MODE_LOAD><<<grid, block, 0, st>>>(args, args.bra.prim);
+            if (nk == 1) jk_kernel<LA, LB, LC, LD, R, 1, MODE_LOAD><<<grid, block, 0, st>>>(args, args.bra.aos);
+            else jk_kernel<LA, LB, LC, LD, R, 2, MODE_LOAD><<<grid, block, 0, st>>>(args, args.bra.aos);
             return;
         }
     }

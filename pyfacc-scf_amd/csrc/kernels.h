@@ -18,6 +18,9 @@ void launch_onee_batch(int nmol, int max_nsh, const Shell *d_shells, const int *
 // ---------------------------------------------------------------- pairs
 // Per primitive pair data, in this field order, stored structure-of-arrays.
 enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_O2P, PF_BND, PF_NFIELD };
+// stride of one primitive in the pair-major copy (a bra pair's primitive
+// fields are read by wavefront-uniform scalar loads, contiguous there)
+constexpr int PF_AOS = 16;
 // A ket primitive pair is visited for a bra pair only when its bound PF_BND =
 // |K|/sqrt(q) times the bra's largest primitive bound reaches PRIM_EPS
 // (|(ab|cd)_prim| is at most the product of the two bounds times the
@@ -37,6 +40,7 @@ struct PairView {
     const R *Q = nullptr;                    // [n] Schwarz bound
     const R *AB = nullptr;                   // [3][npad]  A - B
     const R *prim = nullptr;                 // [kp][PF_NFIELD][npad], per pair by bound |K|/sqrt(p) descending
+    const R *aos = nullptr;                  // the same, [npad][kp][PF_AOS]: one pair's primitives contiguous
     const int *kpn = nullptr;                // [npad] primitive pairs that the ERI kernels visit
     R peps = R(0);                           // ket-primitive cut against the bra's bound (0: none)
 };

@@ -109,6 +109,18 @@ __global__ void __launch_bounds__(1024) max_reduce_kernel(int n, const double *x
 
 int round64(int n) { return ((n + 63) / 64) * 64; }
 
+// prim [kp][PF_NFIELD][npad] -> aos [npad][kp][PF_AOS] (padding fields 0)
+template <typename R>
+__global__ void __launch_bounds__(256) prim_aos_kernel(int64_t total, int npad, int kp, const R *__restrict__ prim,
+                                                       R *__restrict__ aos) {
+    const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (e >= total) return;
+    const int f = (int)(e % PF_AOS);
+    const int64_t r = e / PF_AOS;
+    const int u = (int)(r % kp), i = (int)(r / kp);
+    aos[e] = f < PF_NFIELD ? prim[((int64_t)u * PF_NFIELD + f) * npad + i] : R(0);
+}
+
 // Gradient primitive-pair sets, primitives in shell order (x over A, y over
 // B), one thread per pair slot: w (P - A, K 2 alpha), rw (P - B, K 2 beta),
 // r (P - B, K).  Padding slots and primitives: p = 1, 1/2p = 1/2, K = 0.
@@ -192,12 +204,18 @@ void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shel
                                                               ps.d_prim.p, ps.d_AB.p, ps.d_sa.p, ps.d_sb.p,
                                                               ps.d_oa.p, ps.d_ob.p, ps.d_kpn.p, bmax.p);
     PF_HIP(hipGetLastError());
+    const int64_t naos = (int64_t)ps.npad * kp * PF_AOS;
+    ps.d_aos.alloc((size_t)naos);
+    prim_aos_kernel<double><<<(unsigned)((naos + 255) / 256), 256, 0, st>>>(naos, ps.npad, kp, ps.d_prim.p, ps.d_aos.p);
     if (want32) {
         ps.f_AB.alloc(3 * (size_t)ps.npad);
         ps.f_prim.alloc(nprim);
         to_f32_kernel<<<(unsigned)((3 * (size_t)ps.npad + 255) / 256), 256, 0, st>>>(3 * (size_t)ps.npad, ps.d_AB.p,
                                                                                      ps.f_AB.p);
         to_f32_kernel<<<(unsigned)((nprim + 255) / 256), 256, 0, st>>>(nprim, ps.d_prim.p, ps.f_prim.p);
+        ps.f_aos.alloc((size_t)naos);
+        prim_aos_kernel<float><<<(unsigned)((naos + 255) / 256), 256, 0, st>>>(naos, ps.npad, kp, ps.f_prim.p,
+                                                                               ps.f_aos.p);
     }
     double m = 0.0;
     if (ps.n) {
