@@ -27,6 +27,7 @@ struct GradArgs {
     const double *bra_r = nullptr;    // bra reversed (P - B, K)
     const double *ket_w = nullptr;    // ket prim data, K weighted by 2 gamma(first shell)
     const double *ket_r = nullptr;    // ket reversed (Q - D, K): its K is the unweighted one
+    const double *bra_rec = nullptr;  // bra records [npad][kp][PF_AOS] (GR_ fields): one wide scalar load each
     int same = 0;
     double tau = 0;
     int nbf = 0, nsh = 0;
@@ -333,13 +334,12 @@ __global__ void __launch_bounds__(64 * GRAD_W) grad2e_kernel(GradArgs g, int ib,
                         const double Qx = q[PF_PX * K.npad], Qy = q[PF_PY * K.npad], Qz = q[PF_PZ * K.npad];
                         const double QCx = q[PF_AX * K.npad], QCy = q[PF_AY * K.npad], QCz = q[PF_AZ * K.npad];
                         const double Kg = q[PF_K * K.npad], Kk = g.ket_r[ko + PF_K * K.npad];
+                        const double *brec = g.bra_rec + (size_t)i * B.kp * PF_AOS;
                         for (int v = 0; v < B.kp; ++v) {
-                            const size_t bo = (size_t)v * PF_NFIELD * B.npad + i;
-                            const double *w = g.bra_w + bo;
-                            DF::prim(w[PF_P * B.npad], w[PF_O2P * B.npad], w[PF_PX * B.npad], w[PF_PY * B.npad],
-                                     w[PF_PZ * B.npad], w[PF_AX * B.npad], w[PF_AY * B.npad], w[PF_AZ * B.npad],
-                                     g.bra_r[bo + PF_K * B.npad], w[PF_K * B.npad], g.bra_rw[bo + PF_K * B.npad], qq,
-                                     o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, Kg, lds_boys, acc);
+                            const double *w = brec + (size_t)v * PF_AOS;
+                            DF::prim(w[GR_P], w[GR_O2P], w[GR_PX], w[GR_PY], w[GR_PZ], w[GR_AX], w[GR_AY], w[GR_AZ],
+                                     w[GR_K], w[GR_KA], w[GR_KB], qq, o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, Kg, lds_boys,
+                                     acc);
                         }
                     }
                     DF::contract(acc, gam, ABx, ABy, ABz, dA, dB, dC);

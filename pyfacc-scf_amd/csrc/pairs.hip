@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) prim_aos_kernel(int64_t total, int npad, 
 // r (P - B, K).  Padding slots and primitives: p = 1, 1/2p = 1/2, K = 0.
 __global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict__ sh, const int *__restrict__ sa,
                                                          const int *__restrict__ sb, int n, int npad, int kp,
-                                                         double *w, double *rw, double *r) {
+                                                         double *w, double *rw, double *r, double *rec) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= npad) return;
     const double kfac = sqrt(2.0) * pow(M_PI, 1.25);
@@ -153,6 +153,17 @@ __global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict
                 crw[PF_K * npad] = 2.0 * b * K;
                 cr[PF_K * npad] = K;
                 cw[PF_BND * npad] = crw[PF_BND * npad] = cr[PF_BND * npad] = 0.0;
+                double *g = rec + ((size_t)i * kp + u) * PF_AOS;
+                g[GR_P] = p;
+                g[GR_O2P] = 0.5 / p;
+                for (int d = 0; d < 3; ++d) {
+                    g[GR_PX + d] = (a * A.xyz[d] + b * B.xyz[d]) / p;
+                    g[GR_AX + d] = g[GR_PX + d] - A.xyz[d];
+                }
+                g[GR_K] = K;
+                g[GR_KA] = 2.0 * a * K;
+                g[GR_KB] = 2.0 * b * K;
+                for (int f = GR_KB + 1; f < PF_AOS; ++f) g[f] = 0.0;
             }
     }
     for (; u < kp; ++u) {
@@ -162,15 +173,20 @@ __global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict
             c[PF_P * npad] = 1.0;
             c[PF_O2P * npad] = 0.5;
         }
+        double *g = rec + ((size_t)i * kp + u) * PF_AOS;
+        for (int f = 0; f < PF_AOS; ++f) g[f] = 0.0;
+        g[GR_P] = 1.0;
+        g[GR_O2P] = 0.5;
     }
 }
 
 }  // namespace
 
-void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *rw, double *r, hipStream_t st) {
+void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *rw, double *r, double *rec,
+                       hipStream_t st) {
     if (ps.npad == 0) return;
     grad_pairs_kernel<<<(ps.npad + 255) / 256, 256, 0, st>>>(d_shells, ps.d_sa.p, ps.d_sb.p, ps.n, ps.npad, ps.kp, w,
-                                                             rw, r);
+                                                             rw, r, rec);
     PF_HIP(hipGetLastError());
 }
 
