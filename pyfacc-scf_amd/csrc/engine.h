@@ -38,11 +38,11 @@ void fill_pairset(PairSet &ps, const std::vector<Shell> &sh, const Shell *d_shel
                   const std::vector<std::pair<int, int>> &list, double kref, double &bound_max, bool want32,
                   hipStream_t st);
 
-// The gradient's three primitive-pair sets of ps (kp * PF_NFIELD * npad
-// doubles each, primitives in shell order): P - A with K 2 alpha, P - B with
-// K 2 beta, P - B with K; and its bra records rec (npad * kp * PF_AOS, GR_ fields).
-void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *rw, double *r, double *rec,
-                       hipStream_t st);
+// The gradient's primitive-pair data of ps, primitives in shell order: the
+// structure-of-arrays sets w (P - A, K 2 alpha) and r (P - A, K), each
+// kp * PF_NFIELD * npad doubles, and the records rec (npad * kp * PF_AOS, GR_
+// fields).
+void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *r, double *rec, hipStream_t st);
 
 struct QuartetClass {
     int bra, ket;  // indices into Engine::pairs_
@@ -191,7 +191,7 @@ class Engine {
     bool have_fin_ = false, uhf_fin_ = false;
     // gradient pair data: 2 alpha weighted, reversed 2 beta weighted, reversed
     bool grad_ready_ = false;
-    std::array<DevBuf<double>, 3> gw_, grw_, gr_, grec_;
+    std::array<DevBuf<double>, 3> gw_, gr_, grec_;
     // one-electron derivative pairs: ordered shell pairs by (la, lb) = (0,0), (0,1), (1,0), (1,1)
     DevBuf<int> g1pairs_;                // the four classes' (a, b) lists, a's then b's per class
     std::array<int, 4> g1n_{}, g1off_{};

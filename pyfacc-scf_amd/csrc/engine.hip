@@ -1051,10 +1051,9 @@ void Engine::setup_grad_pairs() {
         const PairSet &ps = pairs_[c];
         const size_t sz = (size_t)ps.kp * PF_NFIELD * ps.npad;
         gw_[c].alloc(std::max<size_t>(sz, 1));
-        grw_[c].alloc(std::max<size_t>(sz, 1));
         gr_[c].alloc(std::max<size_t>(sz, 1));
         grec_[c].alloc(std::max<size_t>((size_t)ps.npad * ps.kp * PF_AOS, 1));
-        fill_grad_pairset(ps, d_shells_.p, gw_[c].p, grw_[c].p, gr_[c].p, grec_[c].p, st_);
+        fill_grad_pairset(ps, d_shells_.p, gw_[c].p, gr_[c].p, grec_[c].p, st_);
     }
     const int nsh = nshell();
     std::vector<int> flat;
@@ -1121,8 +1120,7 @@ void Engine::gradient(double *grad_out, double tau, double *terms) {
         GradArgs g;
         g.bra = pairs_[bc].view64();
         g.ket = pairs_[kc].view64();
-        g.bra_w = gw_[bc].p; g.bra_rw = grw_[bc].p; g.bra_r = gr_[bc].p; g.ket_w = gw_[kc].p; g.ket_r = gr_[kc].p;
-        g.bra_rec = grec_[bc].p;
+        g.bra_rec = grec_[bc].p; g.ket_w = gw_[kc].p; g.ket_r = gr_[kc].p;
         g.same = bc == kc;
         g.tau = tau;
         g.nbf = n; g.nsh = nsh;

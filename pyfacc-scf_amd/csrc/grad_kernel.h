@@ -22,12 +22,9 @@ namespace pf {
 
 struct GradArgs {
     PairView<double> bra, ket;        // normal pair data (Q, shells, AB, prim)
-    const double *bra_w = nullptr;    // bra prim data, K weighted by 2 alpha(first shell)
-    const double *bra_rw = nullptr;   // bra reversed (P - B, K 2 beta)
-    const double *bra_r = nullptr;    // bra reversed (P - B, K)
-    const double *ket_w = nullptr;    // ket prim data, K weighted by 2 gamma(first shell)
-    const double *ket_r = nullptr;    // ket reversed (Q - D, K): its K is the unweighted one
     const double *bra_rec = nullptr;  // bra records [npad][kp][PF_AOS] (GR_ fields): one wide scalar load each
+    const double *ket_w = nullptr;    // ket prim data in shell order, K weighted by 2 gamma (first shell)
+    const double *ket_r = nullptr;    // ket prim data in shell order, K unweighted
     int same = 0;
     double tau = 0;
     int nbf = 0, nsh = 0;
@@ -61,6 +58,33 @@ __device__ __forceinline__ void contracted_block(const double *__restrict__ bp, 
             E::template prim<true, MT>(b[PF_P * bnpad], b[PF_O2P * bnpad], b[PF_PX * bnpad], b[PF_PY * bnpad],
                                        b[PF_PZ * bnpad], b[PF_AX * bnpad], b[PF_AY * bnpad], b[PF_AZ * bnpad],
                                        b[PF_K * bnpad], qq, o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, tab, acc);
+        }
+    }
+    E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+// The same with the bra pair's primitives from its records (bi uniform: one
+// wide scalar load per primitive), K field KF and displacement fields DX
+// (GR_AX: P - A, GR_BX: P - B).
+template <int LA, int LB, int LC, int LD, int MT, int KF, int DX>
+__device__ __forceinline__ void contracted_block_rec(const double *__restrict__ brec, int bi, int kpb,
+                                                     const double *__restrict__ kq, int knpad, int kj, int kpk,
+                                                     double ABx, double ABy, double ABz, double CDx, double CDy,
+                                                     double CDz, const double *__restrict__ tab, double *out) {
+    using E = Eri<LA, LB, LC, LD, double>;
+    double acc[E::NACC];
+#pragma unroll
+    for (int k = 0; k < E::NACC; ++k) acc[k] = 0.0;
+    const double *bpair = brec + (size_t)bi * kpb * PF_AOS;
+    for (int u = 0; u < kpk; ++u) {
+        const double *q = kq + (size_t)u * PF_NFIELD * knpad + kj;
+        const double qq = q[PF_P * knpad], Qx = q[PF_PX * knpad], Qy = q[PF_PY * knpad], Qz = q[PF_PZ * knpad];
+        const double QCx = q[PF_AX * knpad], QCy = q[PF_AY * knpad], QCz = q[PF_AZ * knpad], Kk = q[PF_K * knpad];
+        const double o2q = q[PF_O2P * knpad];
+        for (int v = 0; v < kpb; ++v) {
+            const double *b = bpair + (size_t)v * PF_AOS;
+            E::template prim<true, MT>(b[GR_P], b[GR_O2P], b[GR_PX], b[GR_PY], b[GR_PZ], b[DX], b[DX + 1], b[DX + 2],
+                                       b[KF], qq, o2q, Qx, Qy, Qz, QCx, QCy, QCz, Kk, tab, acc);
         }
     }
     E::hrr(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
@@ -347,8 +371,9 @@ __global__ void __launch_bounds__(64 * GRAD_W) grad2e_kernel(GradArgs g, int ib,
                     // ---- centre A: (a+1, b | c d) with 2 alpha weights, minus (a-1, b | c d)
                     {
                         double e1[NAp * NB * NC * ND];
-                        contracted_block<LA + 1, LB, LC, LD, MT>(g.bra_w, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, ABx,
-                                                                 ABy, ABz, CDx, CDy, CDz, lds_boys, e1);
+                        contracted_block_rec<LA + 1, LB, LC, LD, MT, GR_KA, GR_AX>(g.bra_rec, i, B.kp, K.prim, K.npad,
+                                                                                   j, K.kp, ABx, ABy, ABz, CDx, CDy,
+                                                                                   CDz, lds_boys, e1);
 #pragma unroll
                         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -360,8 +385,9 @@ __global__ void __launch_bounds__(64 * GRAD_W) grad2e_kernel(GradArgs g, int ib,
                             }
                         if constexpr (LA == 1) {
                             double e0[NB * NC * ND];
-                            contracted_block<0, LB, LC, LD, MT>(B.prim, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, ABx,
-                                                                ABy, ABz, CDx, CDy, CDz, lds_boys, e0);
+                            contracted_block_rec<0, LB, LC, LD, MT, GR_K, GR_AX>(g.bra_rec, i, B.kp, K.prim, K.npad, j,
+                                                                                 K.kp, ABx, ABy, ABz, CDx, CDy, CDz,
+                                                                                 lds_boys, e0);
 #pragma unroll
                             for (int x = 0; x < 3; ++x)
 #pragma unroll
@@ -371,8 +397,9 @@ __global__ void __launch_bounds__(64 * GRAD_W) grad2e_kernel(GradArgs g, int ib,
                     // ---- centre B: reversed pair (b+1, a | c d) with 2 beta weights, minus (b-1, a | c d)
                     {
                         double e1[NBp * NA * NC * ND];
-                        contracted_block<LB + 1, LA, LC, LD, MT>(g.bra_rw, B.npad, i, B.kp, K.prim, K.npad, j, K.kp,
-                                                                 -ABx, -ABy, -ABz, CDx, CDy, CDz, lds_boys, e1);
+                        contracted_block_rec<LB + 1, LA, LC, LD, MT, GR_KB, GR_BX>(g.bra_rec, i, B.kp, K.prim, K.npad,
+                                                                                   j, K.kp, -ABx, -ABy, -ABz, CDx,
+                                                                                   CDy, CDz, lds_boys, e1);
 #pragma unroll
                         for (int ib2 = 0; ib2 < NB; ++ib2)
 #pragma unroll
@@ -386,8 +413,9 @@ __global__ void __launch_bounds__(64 * GRAD_W) grad2e_kernel(GradArgs g, int ib,
                             }
                         if constexpr (LB == 1) {
                             double e0[NA * NC * ND];
-                            contracted_block<0, LA, LC, LD, MT>(g.bra_r, B.npad, i, B.kp, K.prim, K.npad, j, K.kp, -ABx,
-                                                                -ABy, -ABz, CDx, CDy, CDz, lds_boys, e0);
+                            contracted_block_rec<0, LA, LC, LD, MT, GR_K, GR_BX>(g.bra_rec, i, B.kp, K.prim, K.npad, j,
+                                                                                 K.kp, -ABx, -ABy, -ABz, CDx, CDy, CDz,
+                                                                                 lds_boys, e0);
 #pragma unroll
                             for (int x = 0; x < 3; ++x)
 #pragma unroll

@@ -22,8 +22,8 @@ enum PrimField { PF_P = 0, PF_PX, PF_PY, PF_PZ, PF_AX, PF_AY, PF_AZ, PF_K, PF_O2
 // fields are read by wavefront-uniform scalar loads, contiguous there)
 constexpr int PF_AOS = 16;
 // fields of the gradient's bra records ([pair][primitive][PF_AOS], shell
-// order): p, 1/2p, P, P - A, K, 2 alpha K, 2 beta K
-enum GradRecField { GR_P = 0, GR_O2P, GR_PX, GR_PY, GR_PZ, GR_AX, GR_AY, GR_AZ, GR_K, GR_KA, GR_KB };
+// order): p, 1/2p, P, P - A, K, 2 alpha K, 2 beta K, P - B
+enum GradRecField { GR_P = 0, GR_O2P, GR_PX, GR_PY, GR_PZ, GR_AX, GR_AY, GR_AZ, GR_K, GR_KA, GR_KB, GR_BX, GR_BY, GR_BZ };
 // A ket primitive pair is visited for a bra pair only when its bound PF_BND =
 // |K|/sqrt(q) times the bra's largest primitive bound reaches PRIM_EPS
 // (|(ab|cd)_prim| is at most the product of the two bounds times the

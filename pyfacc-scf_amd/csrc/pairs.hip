@@ -121,12 +121,14 @@ __global__ void __launch_bounds__(256) prim_aos_kernel(int64_t total, int npad, 
     aos[e] = f < PF_NFIELD ? prim[((int64_t)u * PF_NFIELD + f) * npad + i] : R(0);
 }
 
-// Gradient primitive-pair sets, primitives in shell order (x over A, y over
-// B), one thread per pair slot: w (P - A, K 2 alpha), rw (P - B, K 2 beta),
-// r (P - B, K).  Padding slots and primitives: p = 1, 1/2p = 1/2, K = 0.
+// Gradient primitive-pair data, primitives in shell order (x over A, y over
+// B), one thread per pair slot: the structure-of-arrays sets w (P - A, K
+// 2 alpha) and r (P - A, K), read per lane when the pair is a ket, and the
+// records rec (GR_ fields), read by scalar loads when it is the bra.
+// Padding slots and primitives: p = 1, 1/2p = 1/2, K = 0.
 __global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict__ sh, const int *__restrict__ sa,
                                                          const int *__restrict__ sb, int n, int npad, int kp,
-                                                         double *w, double *rw, double *r, double *rec) {
+                                                         double *w, double *r, double *rec) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= npad) return;
     const double kfac = sqrt(2.0) * pow(M_PI, 1.25);
@@ -140,35 +142,26 @@ __global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict
                 const double a = A.exp[x], b = B.exp[y], p = a + b;
                 const double K = kfac * A.coef[x] * B.coef[y] * exp(-a * b / p * r2) / p;
                 const size_t o = (size_t)u * PF_NFIELD * npad + i;
-                double *cw = w + o, *crw = rw + o, *cr = r + o;
-                cw[PF_P * npad] = crw[PF_P * npad] = cr[PF_P * npad] = p;
-                cw[PF_O2P * npad] = crw[PF_O2P * npad] = cr[PF_O2P * npad] = 0.5 / p;
+                double *cw = w + o, *cr = r + o;
+                double *g = rec + ((size_t)i * kp + u) * PF_AOS;
+                cw[PF_P * npad] = cr[PF_P * npad] = g[GR_P] = p;
+                cw[PF_O2P * npad] = cr[PF_O2P * npad] = g[GR_O2P] = 0.5 / p;
                 for (int d = 0; d < 3; ++d) {
                     const double P = (a * A.xyz[d] + b * B.xyz[d]) / p;
-                    cw[(PF_PX + d) * npad] = crw[(PF_PX + d) * npad] = cr[(PF_PX + d) * npad] = P;
-                    cw[(PF_AX + d) * npad] = P - A.xyz[d];
-                    crw[(PF_AX + d) * npad] = cr[(PF_AX + d) * npad] = P - B.xyz[d];
+                    cw[(PF_PX + d) * npad] = cr[(PF_PX + d) * npad] = g[GR_PX + d] = P;
+                    cw[(PF_AX + d) * npad] = cr[(PF_AX + d) * npad] = g[GR_AX + d] = P - A.xyz[d];
+                    g[GR_BX + d] = P - B.xyz[d];
                 }
-                cw[PF_K * npad] = 2.0 * a * K;
-                crw[PF_K * npad] = 2.0 * b * K;
-                cr[PF_K * npad] = K;
-                cw[PF_BND * npad] = crw[PF_BND * npad] = cr[PF_BND * npad] = 0.0;
-                double *g = rec + ((size_t)i * kp + u) * PF_AOS;
-                g[GR_P] = p;
-                g[GR_O2P] = 0.5 / p;
-                for (int d = 0; d < 3; ++d) {
-                    g[GR_PX + d] = (a * A.xyz[d] + b * B.xyz[d]) / p;
-                    g[GR_AX + d] = g[GR_PX + d] - A.xyz[d];
-                }
-                g[GR_K] = K;
-                g[GR_KA] = 2.0 * a * K;
+                cw[PF_K * npad] = g[GR_KA] = 2.0 * a * K;
+                cr[PF_K * npad] = g[GR_K] = K;
                 g[GR_KB] = 2.0 * b * K;
-                for (int f = GR_KB + 1; f < PF_AOS; ++f) g[f] = 0.0;
+                cw[PF_BND * npad] = cr[PF_BND * npad] = 0.0;
+                for (int f = GR_BZ + 1; f < PF_AOS; ++f) g[f] = 0.0;
             }
     }
     for (; u < kp; ++u) {
         const size_t o = (size_t)u * PF_NFIELD * npad + i;
-        for (double *c : {w + o, rw + o, r + o}) {
+        for (double *c : {w + o, r + o}) {
             for (int f = 0; f < PF_NFIELD; ++f) c[f * npad] = 0.0;
             c[PF_P * npad] = 1.0;
             c[PF_O2P * npad] = 0.5;
@@ -182,11 +175,10 @@ __global__ void __launch_bounds__(256) grad_pairs_kernel(const Shell *__restrict
 
 }  // namespace
 
-void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *rw, double *r, double *rec,
-                       hipStream_t st) {
+void fill_grad_pairset(const PairSet &ps, const Shell *d_shells, double *w, double *r, double *rec, hipStream_t st) {
     if (ps.npad == 0) return;
     grad_pairs_kernel<<<(ps.npad + 255) / 256, 256, 0, st>>>(d_shells, ps.d_sa.p, ps.d_sb.p, ps.n, ps.npad, ps.kp, w,
-                                                             rw, r, rec);
+                                                             r, rec);
     PF_HIP(hipGetLastError());
 }
 
