@@ -9,10 +9,11 @@
 //   d/dA_x [a] = 2 alpha [a + 1_x] - a_x [a - 1_x]
 // so the derivative block of a contracted quartet is an ordinary contracted
 // ERI of one higher angular momentum on the differentiated centre, computed
-// with pair coefficients weighted by 2 alpha (a separate primitive-pair data
-// set), minus the lower block.  Centre B uses the reversed pair (b a|, centre
-// C the swapped quartet (cd|ab), centre D translational invariance
-// dD = -(dA + dB + dC).  The reference has no gradients; this extends the
+// with pair coefficients weighted by 2 alpha, minus the lower block; centre D
+// follows from translational invariance, dD = -(dA + dB + dC).  Classes with
+// b and d s-type get all three blocks from one recurrence (DFused); the
+// others take centre B from the reversed pair (b a| and centre C from the
+// swapped quartet (cd|ab).  The reference has no gradients; this extends the
 // ERI path of libint_f.f90:303-625 / rscf.f90:118-137.
 #pragma once
 #include "eri_core.h"
