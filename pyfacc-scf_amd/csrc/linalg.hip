@@ -364,33 +364,45 @@ __device__ __forceinline__ void atomic_max_nonneg(float *p, float v) {
     atomicMax(reinterpret_cast<unsigned int *>(p), __float_as_uint(v));
 }
 template <typename R>
-__global__ void __launch_bounds__(64) shell_dmax_kernel(int nsh, int nbf, const int *sh_bf, const int *sh_nf,
-                                                        const double *D0, const double *D1, const double *D2, R *Dsh,
-                                                        R *dmax) {
-    const int a = blockIdx.y, b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) shell_dmax_kernel(int nsh, int nbf, const int *sh_bf, const int *sh_nf,
+                                                         const double *D0, const double *D1, const double *D2, R *Dsh,
+                                                         R *dmax) {
+    // one wavefront per shell row a, its lanes over the column shells b; one
+    // device atomic per workgroup of four rows
+    __shared__ R wmax[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a = blockIdx.x * 4 + w;
     double m = 0.0;
-    if (b < nsh) {
-        for (int i = 0; i < sh_nf[a]; ++i)
-            for (int j = 0; j < sh_nf[b]; ++j) {
-                const size_t ij = (size_t)(sh_bf[a] + i) * nbf + sh_bf[b] + j;
-                m = fmax(m, fabs(D0[ij]));
-                if (D1) m = fmax(m, fabs(D1[ij]));
-                if (D2) m = fmax(m, fabs(D2[ij]));
-            }
-        Dsh[(size_t)a * nsh + b] = R(m);
+    if (a < nsh) {
+        const int fa = sh_bf[a], na = sh_nf[a];
+        for (int b = lane; b < nsh; b += 64) {
+            const int fb = sh_bf[b], nb = sh_nf[b];
+            double mb = 0.0;
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j) {
+                    const size_t ij = (size_t)(fa + i) * nbf + fb + j;
+                    mb = fmax(mb, fabs(D0[ij]));
+                    if (D1) mb = fmax(mb, fabs(D1[ij]));
+                    if (D2) mb = fmax(mb, fabs(D2[ij]));
+                }
+            Dsh[(size_t)a * nsh + b] = R(mb);
+            m = fmax(m, mb);
+        }
     }
-    R w = R(m);
+    R v = R(m);
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) w = max(w, __shfl_xor(w, o, 64));
-    if (threadIdx.x == 0) atomic_max_nonneg(dmax, R(2) * w);
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    if (lane == 0) wmax[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomic_max_nonneg(dmax, R(2) * max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
 }
 template <typename R>
 void dev_shell_dmax(int nsh, int nbf, const int *sh_bf, const int *sh_nf, const double *D0, const double *D1, R *Dsh,
                     R *dmax, hipStream_t st, const double *D2) {
-    dim3 grid((nsh + 63) / 64, nsh);
     if constexpr (std::is_same<R, double>::value) dev_fill(1, 0.0, dmax, st);
     else dev_fillf(1, 0.0f, dmax, st);
-    shell_dmax_kernel<R><<<grid, 64, 0, st>>>(nsh, nbf, sh_bf, sh_nf, D0, D1, D2, Dsh, dmax);
+    if (nsh <= 0) return;
+    shell_dmax_kernel<R><<<(nsh + 3) / 4, 256, 0, st>>>(nsh, nbf, sh_bf, sh_nf, D0, D1, D2, Dsh, dmax);
     PF_HIP(hipGetLastError());
 }
 template void dev_shell_dmax<double>(int, int, const int *, const int *, const double *, const double *, double *,
